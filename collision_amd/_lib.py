@@ -1,0 +1,119 @@
+"""ctypes binding of libcollision_amd.so (the C ABI in include/collision_amd.h).
+
+Loading fails loudly if the library has not been built: there is no CPU
+fallback for the collision step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libcollision_amd.so")
+HEADER = os.path.join(ROOT, "include", "collision_amd.h")
+
+CS_OK = 0
+CS_ERR_ARG = 1
+CS_ERR_NONFINITE = 2
+CS_ERR_UNSUPPORTED = 3
+CS_ERR_HIP = 4
+CS_ERR_NOMEM = 5
+CS_ERR_NO_DEVICE = 6
+CS_ERR_INTERNAL = 7
+CS_WARN_ZONE_CAP = 16
+
+STATUS_NAMES = {0: "CS_OK", 1: "CS_ERR_ARG", 2: "CS_ERR_NONFINITE", 3: "CS_ERR_UNSUPPORTED",
+                4: "CS_ERR_HIP", 5: "CS_ERR_NOMEM", 6: "CS_ERR_NO_DEVICE", 7: "CS_ERR_INTERNAL",
+                16: "CS_WARN_ZONE_CAP"}
+PASS_NAMES = {0: "proximity", 1: "ccd", 2: "zone"}
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("thickness", C.c_double), ("round_eps", C.c_double),
+                ("spring_k", C.c_double), ("mass", C.c_double), ("friction", C.c_double),
+                ("restitution", C.c_double), ("dom_lo", C.c_double * 3),
+                ("dom_hi", C.c_double * 3), ("max_ccd_passes", C.c_int32),
+                ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
+                ("log_pairs", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [("nv", C.c_int32), ("nt", C.c_int32), ("nb", C.c_int32),
+                ("tris", C.POINTER(C.c_int32)), ("tri_surf", C.POINTER(C.c_int32)),
+                ("tri_len0", C.POINTER(C.c_double)), ("bonds", C.POINTER(C.c_int32)),
+                ("bond_len0", C.POINTER(C.c_double)), ("is_fixed", C.POINTER(C.c_uint8))]
+
+
+class PassInfo(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("candidates", C.c_int64),
+                ("narrow_pairs", C.c_int64), ("hit_pairs", C.c_int64),
+                ("hit_subtests", C.c_int64), ("ms_broad", C.c_double),
+                ("ms_narrow", C.c_double), ("ms_response", C.c_double)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_passes", C.c_int32), ("n_ccd_passes", C.c_int32),
+                ("n_zone_passes", C.c_int32), ("has_collision", C.c_int32),
+                ("n_zones", C.c_int32), ("strain_sweeps", C.c_int32),
+                ("strain_edges", C.c_int32), ("degenerate_edges", C.c_int32),
+                ("pad", C.c_int32), ("candidates", C.c_int64), ("narrow_pairs", C.c_int64),
+                ("hit_pairs", C.c_int64), ("hit_subtests", C.c_int64),
+                ("ms_total", C.c_double), ("ms_broad", C.c_double), ("ms_narrow", C.c_double),
+                ("ms_response", C.c_double), ("ms_zone", C.c_double), ("ms_strain", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+def declared_symbols(header: str = HEADER):
+    """Function names declared in the public header."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cs_[a-z_0-9]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with collision_amd.build.build() "
+                           "(there is no CPU fallback for the collision step)")
+    L = C.CDLL(path)
+    P = C.POINTER
+    vp, i32, i64, d, u8 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint8
+    sig = {
+        "cs_default_params": ([P(Params)], None),
+        "cs_create": ([P(vp), P(Params)], C.c_int),
+        "cs_destroy": ([vp], C.c_int),
+        "cs_set_params": ([vp, P(Params)], C.c_int),
+        "cs_get_params": ([vp, P(Params)], C.c_int),
+        "cs_set_domain_boundary": ([vp, P(d), P(d)], C.c_int),
+        "cs_assemble": ([vp, P(MeshView)], C.c_int),
+        "cs_set_state": ([vp, P(d), P(d), P(u8)], C.c_int),
+        "cs_get_state": ([vp, P(d), P(d), P(d), P(d), P(u8)], C.c_int),
+        "cs_record_origin_position": ([vp], C.c_int),
+        "cs_predict": ([vp], C.c_int),
+        "cs_resolve": ([vp, P(StepReport)], C.c_int),
+        "cs_step": ([vp, P(StepReport)], C.c_int),
+        "cs_has_collision": ([vp], C.c_int),
+        "cs_synchronize": ([vp], C.c_int),
+        "cs_pass_count": ([vp], i32),
+        "cs_pass_infos": ([vp, P(PassInfo), i32], C.c_int),
+        "cs_pair_log_count": ([vp], i64),
+        "cs_pair_log": ([vp, P(i32), i64], C.c_int),
+        "cs_zone_labels": ([vp, P(i32)], C.c_int),
+        "cs_last_error": ([vp], C.c_char_p),
+        "cs_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L

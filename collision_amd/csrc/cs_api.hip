@@ -1,0 +1,636 @@
+// C ABI and host orchestration of the collision step (see
+// include/collision_amd.h).  The pass structure is the reference's
+// resolveCollision (dcollid.cpp:317-362): average velocity, one proximity
+// pass, up to five CCD passes, impact-zone passes while collisions remain,
+// domain box, final position, strain limiting, final velocity.  Every pass is
+//   element boxes -> (proximity pass only: LBVH build) -> LBVH refit ->
+//   traversal (compacted candidate stream) -> narrow phase (hit records) ->
+//   canonical segmented sums -> Jacobi apply [-> zone rigid update]
+// with all state resident in device memory; the host only reads a small
+// counter block per pass to decide whether another pass runs and whether a
+// buffer must grow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/collision_amd.h"
+#include "cs_internal.h"
+
+using namespace cs;
+
+namespace {
+
+constexpr int CNT_ERR = 15;
+constexpr int CNT_STRAIN_SWEEPS = 11;
+constexpr int COUNTER_INTS = 32;  // ints 0..15, u64 candidate count at ints 16..17
+std::string g_create_error;
+
+}  // namespace
+
+struct cs_solver {
+  cs_params P{};
+  hipStream_t st = nullptr;
+  DevState S;
+  std::string err;
+  bool assembled = false;
+  bool state_set = false;
+  int has_collision = 0;
+  int* h_counters = nullptr;  // pinned mirror of the counter block
+  std::vector<cs_pass_info> passes;
+  int64_t plog_rows = 0;
+  std::vector<hipEvent_t> events;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int fail(cs_solver* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(h, expr)                                                                    \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return fail((h), CS_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, sizeof(T) * count);
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+void free_dev(DevState& S) {
+  dfree(S.x); dfree(S.x_old); dfree(S.vel); dfree(S.av);
+  dfree(S.fixed); dfree(S.has); dfree(S.uf);
+  dfree(S.ev); dfree(S.eflags); dfree(S.len0); dfree(S.ebox);
+  dfree(S.bounds); dfree(S.mcode); dfree(S.mid); dfree(S.mcode_tmp); dfree(S.mid_tmp);
+  dfree(S.sort_hist); dfree(S.nodes); dfree(S.leaves); dfree(S.parent); dfree(S.lparent);
+  dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
+  dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
+  dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
+  dfree(S.zkey); dfree(S.zkey_tmp); dfree(S.zval); dfree(S.zval_tmp); dfree(S.zsize);
+  dfree(S.lvl_ptr); dfree(S.lvl_elems); dfree(S.plog); dfree(S.counters);
+  S.err = nullptr;
+  S.cand_count = nullptr;
+  S = DevState();
+}
+
+hipError_t alloc_hits(DevState& S, int cap) {
+  dfree(S.hits); dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
+  dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
+  S.hit_cap = cap;
+  CS_HIP_RET(dalloc(&S.hits, cap));
+  CS_HIP_RET(dalloc(&S.hkey, cap));
+  CS_HIP_RET(dalloc(&S.hkey_tmp, cap));
+  CS_HIP_RET(dalloc(&S.hidx, cap));
+  CS_HIP_RET(dalloc(&S.hidx_tmp, cap));
+  CS_HIP_RET(dalloc(&S.rkey, 4 * (size_t)cap));
+  CS_HIP_RET(dalloc(&S.rkey_tmp, 4 * (size_t)cap));
+  CS_HIP_RET(dalloc(&S.rval, 4 * (size_t)cap));
+  CS_HIP_RET(dalloc(&S.rval_tmp, 4 * (size_t)cap));
+  return hipSuccess;
+}
+
+hipError_t alloc_sort_hist(DevState& S) {
+  const int nmax = std::max({S.ne, S.nv, 4 * S.hit_cap, 1});
+  const size_t words = radix_hist_words(nmax);
+  if (words <= S.sort_hist_words) return hipSuccess;
+  dfree(S.sort_hist);
+  S.sort_hist_words = words;
+  return dalloc(&S.sort_hist, words);
+}
+
+void to_dvec4(const double* src, int n, std::vector<dvec4>& out) {
+  out.resize((size_t)n);
+  for (int i = 0; i < n; ++i) out[i] = dvec4{src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0};
+}
+
+void from_dvec4(const std::vector<dvec4>& in, double* dst) {
+  for (size_t i = 0; i < in.size(); ++i) {
+    dst[3 * i] = in[i].x;
+    dst[3 * i + 1] = in[i].y;
+    dst[3 * i + 2] = in[i].z;
+  }
+}
+
+hipEvent_t next_event(cs_solver* h) {
+  if (h->ev_used == h->events.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->events.push_back(e);
+  }
+  return h->events[h->ev_used++];
+}
+
+void mark(cs_solver* h, hipEvent_t* slot) {
+  *slot = next_event(h);
+  if (*slot) (void)hipEventRecord(*slot, h->st);
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (a && b && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms;
+  return 0.f;
+}
+
+int read_counters(cs_solver* h) {
+  HIPCHK(h, hipMemcpyAsync(h->h_counters, h->S.counters, sizeof(int) * COUNTER_INTS,
+                           hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return CS_OK;
+}
+
+int device_error(cs_solver* h) {
+  const int e = h->h_counters[CNT_ERR];
+  if (e & CS_DEVERR_NONFINITE)
+    return fail(h, CS_ERR_NONFINITE, "non-finite value in the collision step");
+  if (e & CS_DEVERR_STACK) return fail(h, CS_ERR_INTERNAL, "BVH traversal stack overflow");
+  if (e) return fail(h, CS_ERR_INTERNAL, "device error flag set");
+  return CS_OK;
+}
+
+struct PassTimes {
+  hipEvent_t e0, e1, e2, e3;
+};
+
+// One broad + narrow + response pass.  Returns status; *any_hit set.
+int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
+  DevState& S = h->S;
+  const cs_params& P = h->P;
+  const int ccd = kind != CS_PASS_PROXIMITY;
+  PassTimes T{};
+  mark(h, &T.e0);
+  launch_element_boxes(h->st, S, ccd, P.dt);
+  if (kind == CS_PASS_PROXIMITY) HIPCHK(h, build_bvh(h->st, S));
+  HIPCHK(h, refit_bvh(h->st, S));
+  // traversal, re-run with a larger stream if it overflowed
+  int npairs = 0;
+  for (;;) {
+    HIPCHK(h, hipMemsetAsync(S.counters + CNT_PAIRS, 0, sizeof(int), h->st));
+    HIPCHK(h, hipMemsetAsync(S.cand_count, 0, sizeof(unsigned long long), h->st));
+    launch_traverse(h->st, S, S.pair_cap);
+    HIPCHK(h, hipGetLastError());
+    int rc = read_counters(h);
+    if (rc) return rc;
+    npairs = h->h_counters[CNT_PAIRS];
+    if (npairs <= S.pair_cap) break;
+    dfree(S.pairs);
+    S.pair_cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
+    HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
+  }
+  const unsigned long long cand = *(unsigned long long*)(h->h_counters + 16);
+  mark(h, &T.e1);
+  NarrowParams NP;
+  NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
+  NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
+  NP.ccd = ccd;
+  NP.zone_mode = (kind == CS_PASS_ZONE);
+  NP.log_pairs = P.log_pairs;
+  NP.pass_id = pass_id;
+  int nhits = 0;
+  for (;;) {
+    HIPCHK(h, hipMemsetAsync(S.counters + CNT_HITS, 0, sizeof(int) * 3, h->st));
+    launch_narrow(h->st, S, NP, npairs);
+    HIPCHK(h, hipGetLastError());
+    int rc = read_counters(h);
+    if (rc) return rc;
+    nhits = h->h_counters[CNT_HITS];
+    if (nhits <= S.hit_cap) break;
+    // grow and re-run: the narrow phase only writes hit records, counters
+    // and (idempotent) zone unions
+    if (P.log_pairs) {
+      // rows of this pass are written again by the re-run; drop them
+      int keep = (int)h->plog_rows;
+      HIPCHK(h, hipMemcpyAsync(S.counters + CNT_LOG, &keep, sizeof(int), hipMemcpyHostToDevice, h->st));
+    }
+    HIPCHK(h, alloc_hits(S, nhits + nhits / 2 + 1024));
+    HIPCHK(h, alloc_sort_hist(S));
+  }
+  mark(h, &T.e2);
+  HIPCHK(h, apply_hits(h->st, S, nhits));
+  if (kind == CS_PASS_ZONE) {
+    HIPCHK(h, hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), h->st));
+    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt));
+  }
+  mark(h, &T.e3);
+  int rc = read_counters(h);
+  if (rc) return rc;
+  if (P.log_pairs) h->plog_rows = std::min<int64_t>(h->h_counters[CNT_LOG], S.plog_cap);
+  cs_pass_info pi{};
+  pi.kind = kind;
+  pi.index = index;
+  pi.candidates = (int64_t)cand;
+  pi.narrow_pairs = npairs;
+  pi.hit_pairs = h->h_counters[CNT_HIT_PAIRS];
+  pi.hit_subtests = h->h_counters[CNT_HIT_SUB];
+  pi.ms_broad = elapsed(T.e0, T.e1);
+  pi.ms_narrow = elapsed(T.e1, T.e2);
+  pi.ms_response = elapsed(T.e2, T.e3);
+  h->passes.push_back(pi);
+  *any_hit = pi.hit_pairs > 0;
+  return device_error(h);
+}
+
+int check_ready(cs_solver* h) {
+  if (!h) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  if (!h->state_set) return fail(h, CS_ERR_ARG, "cs_set_state has not been called");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  return CS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cs_default_params(cs_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->dt = 1e-3;            // DT (collid.h:19)
+  p->thickness = 1e-4;     // dcollid.cpp:30
+  p->round_eps = 1e-6;     // EPS (collid.h:18)
+  p->spring_k = 1000.0;    // dcollid.cpp:32
+  p->mass = 0.01;          // dcollid.cpp:33
+  p->friction = 0.02;      // dcollid.cpp:34
+  p->restitution = 0.0;    // dcollid.cpp:35
+  for (int i = 0; i < 3; ++i) {
+    p->dom_lo[i] = -1e30;
+    p->dom_hi[i] = 1e30;
+  }
+  p->max_ccd_passes = 5;
+  p->max_zone_passes = 1000;
+  p->max_strain_sweeps = 10;
+  p->log_pairs = 0;
+  p->device = 0;
+}
+
+int cs_create(cs_solver** out, const cs_params* p) {
+  if (!out) return fail(nullptr, CS_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(nullptr, CS_ERR_NO_DEVICE, "no HIP device available");
+  cs_solver* h = new (std::nothrow) cs_solver();
+  if (!h) return fail(nullptr, CS_ERR_NOMEM, "out of host memory");
+  if (p) h->P = *p;
+  else cs_default_params(&h->P);
+  if (h->P.device < 0 || h->P.device >= ndev) {
+    delete h;
+    return fail(nullptr, CS_ERR_ARG, "device ordinal out of range");
+  }
+  if (hipSetDevice(h->P.device) != hipSuccess || hipStreamCreate(&h->st) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_counters, sizeof(int) * COUNTER_INTS) != hipSuccess) {
+    delete h;
+    return fail(nullptr, CS_ERR_HIP, "HIP stream / pinned buffer creation failed");
+  }
+  *out = h;
+  return CS_OK;
+}
+
+int cs_destroy(cs_solver* h) {
+  if (!h) return CS_OK;
+  (void)hipSetDevice(h->P.device);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  free_dev(h->S);
+  for (auto e : h->events) (void)hipEventDestroy(e);
+  if (h->h_counters) (void)hipHostFree(h->h_counters);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return CS_OK;
+}
+
+int cs_set_params(cs_solver* h, const cs_params* p) {
+  if (!h || !p) return CS_ERR_ARG;
+  if (p->device != h->P.device) return fail(h, CS_ERR_ARG, "device cannot change after create");
+  if (!(p->dt >= 0) || !(p->thickness >= 0) || !(p->round_eps >= 0))
+    return fail(h, CS_ERR_ARG, "dt, thickness and round_eps must be >= 0");
+  h->P = *p;
+  return CS_OK;
+}
+
+int cs_get_params(cs_solver* h, cs_params* p) {
+  if (!h || !p) return CS_ERR_ARG;
+  *p = h->P;
+  return CS_OK;
+}
+
+int cs_set_domain_boundary(cs_solver* h, const double* L, const double* U) {
+  if (!h || !L || !U) return CS_ERR_ARG;
+  for (int i = 0; i < 3; ++i) {
+    h->P.dom_lo[i] = L[i];
+    h->P.dom_hi[i] = U[i];
+  }
+  return CS_OK;
+}
+
+int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
+  if (!h || !m) return CS_ERR_ARG;
+  if (m->nv <= 0 || m->nt < 0 || m->nb < 0 || (m->nt > 0 && !m->tris) ||
+      (m->nb > 0 && !m->bonds))
+    return fail(h, CS_ERR_ARG, "invalid mesh view");
+  const int nv = m->nv, nt = m->nt, nb = m->nb, ne = nt + nb;
+  if ((int64_t)ne >= (1ll << 28) || (int64_t)nv >= (1ll << 28))
+    return fail(h, CS_ERR_UNSUPPORTED, "more than 2^28 elements or vertices");
+  std::vector<int4> ev((size_t)std::max(ne, 1));
+  std::vector<uint8_t> ef((size_t)std::max(ne, 1));
+  std::vector<dvec4> l0((size_t)std::max(ne, 1));
+  std::vector<uint8_t> fixed((size_t)nv, 0);
+  if (m->is_fixed) std::memcpy(fixed.data(), m->is_fixed, (size_t)nv);
+  std::vector<uint8_t> used((size_t)nv, 0);
+  for (int e = 0; e < nt; ++e) {
+    int v[3];
+    for (int j = 0; j < 3; ++j) {
+      v[j] = m->tris[3 * e + j];
+      if (v[j] < 0 || v[j] >= nv) return fail(h, CS_ERR_ARG, "triangle vertex id out of range");
+      used[v[j]] = 1;
+    }
+    ev[e] = make_int4(v[0], v[1], v[2], m->tri_surf ? m->tri_surf[e] : 0);
+    ef[e] = EF_TRI | ((fixed[v[0]] || fixed[v[1]] || fixed[v[2]]) ? EF_RIGID : 0);
+    l0[e] = dvec4{m->tri_len0 ? m->tri_len0[3 * e] : 0.0, m->tri_len0 ? m->tri_len0[3 * e + 1] : 0.0,
+                  m->tri_len0 ? m->tri_len0[3 * e + 2] : 0.0, 0.0};
+  }
+  for (int b = 0; b < nb; ++b) {
+    const int e = nt + b;
+    const int v0 = m->bonds[2 * b], v1 = m->bonds[2 * b + 1];
+    if (v0 < 0 || v0 >= nv || v1 < 0 || v1 >= nv)
+      return fail(h, CS_ERR_ARG, "bond vertex id out of range");
+    used[v0] = used[v1] = 1;
+    ev[e] = make_int4(v0, v1, -1, -1);
+    ef[e] = (fixed[v0] || fixed[v1]) ? EF_RIGID : 0;
+    l0[e] = dvec4{m->bond_len0 ? m->bond_len0[b] : 0.0, 0.0, 0.0, 0.0};
+  }
+  for (int v = 0; v < nv; ++v)
+    if (!used[v])
+      return fail(h, CS_ERR_ARG, "every vertex must belong to a triangle or bond");
+  // strain-limit levels: 1 + max level of earlier non-rigid elements that
+  // share a vertex (dcollid.cpp:492-557 visits elements in order)
+  std::vector<int> last((size_t)nv, -1), level((size_t)std::max(ne, 1), -1);
+  int nlev = 0, nsched = 0;
+  for (int e = 0; e < ne; ++e) {
+    if (ef[e] & EF_RIGID) continue;
+    const int np = (ef[e] & EF_TRI) ? 3 : 2;
+    const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
+    int l = 0;
+    for (int i = 0; i < np; ++i) l = std::max(l, last[vv[i]] + 1);
+    for (int i = 0; i < np; ++i) last[vv[i]] = l;
+    level[e] = l;
+    nlev = std::max(nlev, l + 1);
+    ++nsched;
+  }
+  std::vector<int> lptr((size_t)nlev + 1, 0), lel((size_t)std::max(nsched, 1));
+  for (int e = 0; e < ne; ++e)
+    if (level[e] >= 0) lptr[level[e] + 1]++;
+  for (int l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+  {
+    std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+    for (int e = 0; e < ne; ++e)
+      if (level[e] >= 0) lel[fill[level[e]]++] = e;
+  }
+
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  free_dev(h->S);
+  DevState& S = h->S;
+  S.nv = nv; S.nt = nt; S.nb = nb; S.ne = ne;
+  HIPCHK(h, dalloc(&S.x, nv)); HIPCHK(h, dalloc(&S.x_old, nv));
+  HIPCHK(h, dalloc(&S.vel, nv)); HIPCHK(h, dalloc(&S.av, nv));
+  HIPCHK(h, dalloc(&S.fixed, nv)); HIPCHK(h, dalloc(&S.has, nv)); HIPCHK(h, dalloc(&S.uf, nv));
+  HIPCHK(h, dalloc(&S.ev, ne)); HIPCHK(h, dalloc(&S.eflags, ne)); HIPCHK(h, dalloc(&S.len0, ne));
+  HIPCHK(h, dalloc(&S.ebox, ne));
+  HIPCHK(h, dalloc(&S.bounds, 6));
+  HIPCHK(h, dalloc(&S.mcode, ne)); HIPCHK(h, dalloc(&S.mid, ne));
+  HIPCHK(h, dalloc(&S.mcode_tmp, ne)); HIPCHK(h, dalloc(&S.mid_tmp, ne));
+  HIPCHK(h, dalloc(&S.nodes, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.leaves, ne));
+  HIPCHK(h, dalloc(&S.parent, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.lparent, ne));
+  HIPCHK(h, dalloc(&S.rmax, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.arrive, std::max(ne - 1, 1)));
+  S.pair_cap = (int)std::min<int64_t>(8ll * ne + 4096, INT32_MAX / 2);
+  HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
+  HIPCHK(h, alloc_hits(S, std::max(1 << 16, ne / 4)));
+  HIPCHK(h, dalloc(&S.zkey, nv)); HIPCHK(h, dalloc(&S.zkey_tmp, nv));
+  HIPCHK(h, dalloc(&S.zval, nv)); HIPCHK(h, dalloc(&S.zval_tmp, nv)); HIPCHK(h, dalloc(&S.zsize, nv));
+  S.n_levels = nlev;
+  S.n_sched = nsched;
+  HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1)); HIPCHK(h, dalloc(&S.lvl_elems, std::max(nsched, 1)));
+  S.plog_cap = 1 << 20;
+  HIPCHK(h, dalloc(&S.plog, S.plog_cap));
+  HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
+  S.err = S.counters + CNT_ERR;
+  S.cand_count = (unsigned long long*)(S.counters + 16);
+  HIPCHK(h, alloc_sort_hist(S));
+  HIPCHK(h, hipMemcpy(S.ev, ev.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.eflags, ef.data(), ne, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.len0, l0.data(), sizeof(dvec4) * ne, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.fixed, fixed.data(), nv, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.lvl_ptr, lptr.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
+  if (nsched) HIPCHK(h, hipMemcpy(S.lvl_elems, lel.data(), sizeof(int) * nsched, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
+  HIPCHK(h, hipMemset(S.has, 0, nv));
+  h->assembled = true;
+  h->state_set = false;
+  return CS_OK;
+}
+
+int cs_set_state(cs_solver* h, const double* x, const double* vel, const uint8_t* has_collsn) {
+  if (!h || !x || !vel) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  DevState& S = h->S;
+  std::vector<dvec4> buf;
+  to_dvec4(x, S.nv, buf);
+  HIPCHK(h, hipMemcpy(S.x, buf.data(), sizeof(dvec4) * S.nv, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.x_old, buf.data(), sizeof(dvec4) * S.nv, hipMemcpyHostToDevice));
+  to_dvec4(vel, S.nv, buf);
+  HIPCHK(h, hipMemcpy(S.vel, buf.data(), sizeof(dvec4) * S.nv, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(S.av, 0, sizeof(dvec4) * S.nv));
+  if (has_collsn) HIPCHK(h, hipMemcpy(S.has, has_collsn, S.nv, hipMemcpyHostToDevice));
+  else HIPCHK(h, hipMemset(S.has, 0, S.nv));
+  h->state_set = true;
+  return CS_OK;
+}
+
+int cs_get_state(cs_solver* h, double* x, double* vel, double* avg_vel, double* x_old,
+                 uint8_t* has_collsn) {
+  if (!h) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  DevState& S = h->S;
+  std::vector<dvec4> buf((size_t)S.nv);
+  struct {
+    double* dst;
+    dvec4* src;
+  } items[4] = {{x, S.x}, {vel, S.vel}, {avg_vel, S.av}, {x_old, S.x_old}};
+  for (auto& it : items) {
+    if (!it.dst) continue;
+    HIPCHK(h, hipMemcpy(buf.data(), it.src, sizeof(dvec4) * S.nv, hipMemcpyDeviceToHost));
+    from_dvec4(buf, it.dst);
+  }
+  if (has_collsn) HIPCHK(h, hipMemcpy(has_collsn, S.has, S.nv, hipMemcpyDeviceToHost));
+  return CS_OK;
+}
+
+int cs_record_origin_position(cs_solver* h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->S.x_old, h->S.x, sizeof(dvec4) * h->S.nv, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(h, hipMemsetAsync(h->S.has, 0, h->S.nv, h->st));
+  return CS_OK;
+}
+
+int cs_predict(cs_solver* h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  launch_predict(h->st, h->S, h->P.dt);
+  HIPCHK(h, hipGetLastError());
+  return CS_OK;
+}
+
+int cs_resolve(cs_solver* h, cs_step_report* report) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  DevState& S = h->S;
+  const cs_params& P = h->P;
+  cs_step_report R{};
+  h->passes.clear();
+  h->plog_rows = 0;
+  h->ev_used = 0;
+  h->has_collision = 0;
+  hipEvent_t t0, t_strain0, t_end;
+  mark(h, &t0);
+  HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
+  launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
+  launch_average_velocity(h->st, S, P.dt);        // computeAverageVelocity
+  int pass_id = 0;
+  bool any = false;
+  int status = CS_OK;
+  // detectProximity (dcollid.cpp:390-406)
+  rc = run_pass(h, CS_PASS_PROXIMITY, 0, pass_id++, &any);
+  if (rc) status = rc;
+  // detectCollision (dcollid.cpp:430-468)
+  bool is_collision = true;
+  int niter = 1;
+  while (!status && is_collision) {
+    rc = run_pass(h, CS_PASS_CCD, niter - 1, pass_id++, &is_collision);
+    if (rc) { status = rc; break; }
+    if (niter == 1 && is_collision) h->has_collision = 1;
+    R.n_ccd_passes++;
+    if (++niter > P.max_ccd_passes) break;
+  }
+  // computeImpactZone (dcollid.cpp:227-265)
+  if (!status && is_collision) {
+    int zi = 0;
+    while (is_collision) {
+      if (zi >= P.max_zone_passes) { status = CS_WARN_ZONE_CAP; break; }
+      rc = run_pass(h, CS_PASS_ZONE, zi, pass_id++, &is_collision);
+      if (rc) { status = rc; break; }
+      R.n_zones = h->h_counters[CNT_ZONES];
+      R.n_zone_passes++;
+      zi++;
+    }
+  }
+  mark(h, &t_strain0);
+  if (status == CS_OK || status == CS_WARN_ZONE_CAP) {
+    launch_domain_boundary(h->st, S, P.dom_lo, P.dom_hi, P.dt, P.friction, P.thickness);
+    launch_final_position(h->st, S, P.dt);
+    HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_FLAG, 0, sizeof(int) * 4, h->st));
+    HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps, nullptr));
+    launch_final_velocity(h->st, S);
+    HIPCHK(h, hipGetLastError());
+  }
+  mark(h, &t_end);
+  rc = read_counters(h);
+  if (rc) return rc;
+  if (status == CS_OK || status == CS_WARN_ZONE_CAP) {
+    R.strain_sweeps = h->h_counters[CNT_STRAIN_SWEEPS];
+    R.strain_edges = h->h_counters[CNT_STRAIN_EDGES];
+    R.degenerate_edges = h->h_counters[CNT_STRAIN_DEGEN];
+    int e = device_error(h);
+    if (e) status = e;
+  }
+  R.status = status;
+  R.n_passes = (int)h->passes.size();
+  R.has_collision = h->has_collision;
+  for (auto& p : h->passes) {
+    R.candidates += p.candidates;
+    R.narrow_pairs += p.narrow_pairs;
+    R.hit_pairs += p.hit_pairs;
+    R.hit_subtests += p.hit_subtests;
+    R.ms_broad += p.ms_broad;
+    R.ms_narrow += p.ms_narrow;
+    R.ms_response += p.ms_response;
+    if (p.kind == CS_PASS_ZONE) R.ms_zone += p.ms_broad + p.ms_narrow + p.ms_response;
+  }
+  R.ms_total = elapsed(t0, t_end);
+  R.ms_strain = elapsed(t_strain0, t_end);
+  if (report) *report = R;
+  return status;
+}
+
+int cs_step(cs_solver* h, cs_step_report* report) {
+  int rc = cs_predict(h);
+  if (rc) return rc;
+  return cs_resolve(h, report);
+}
+
+int cs_has_collision(cs_solver* h) { return h ? h->has_collision : 0; }
+
+int cs_synchronize(cs_solver* h) {
+  if (!h) return CS_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return CS_OK;
+}
+
+int32_t cs_pass_count(cs_solver* h) { return h ? (int32_t)h->passes.size() : 0; }
+
+int cs_pass_infos(cs_solver* h, cs_pass_info* out, int32_t cap) {
+  if (!h || !out) return CS_ERR_ARG;
+  const int32_t n = std::min<int32_t>(cap, (int32_t)h->passes.size());
+  for (int32_t i = 0; i < n; ++i) out[i] = h->passes[i];
+  return CS_OK;
+}
+
+int64_t cs_pair_log_count(cs_solver* h) { return h ? h->plog_rows : 0; }
+
+int cs_pair_log(cs_solver* h, int32_t* out4, int64_t cap) {
+  if (!h || !out4) return CS_ERR_ARG;
+  const int64_t n = std::min<int64_t>(cap, h->plog_rows);
+  if (n <= 0) return CS_OK;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipMemcpy(out4, h->S.plog, sizeof(int4) * (size_t)n, hipMemcpyDeviceToHost));
+  return CS_OK;
+}
+
+int cs_zone_labels(cs_solver* h, int32_t* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!out) return CS_ERR_ARG;
+  int32_t* d = nullptr;
+  HIPCHK(h, dalloc(&d, h->S.nv));
+  launch_zone_labels(h->st, h->S, d);
+  hipError_t e = hipMemcpyAsync(out, d, sizeof(int32_t) * h->S.nv, hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(h, CS_ERR_HIP, hipGetErrorString(e));
+  return CS_OK;
+}
+
+const char* cs_last_error(cs_solver* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+const char* cs_version(void) { return "collision_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,510 @@
+// Narrow phase for gfx950: static proximity and continuous collision tests
+// over the compacted candidate stream, FP64.
+//
+// One lane per element pair.  The pair's vertices (x_old, avgVel, static
+// flag) are staged once into LDS (thread-major SoA, conflict-free) and every
+// sub-test gathers its four points from there.  Sub-tests follow the
+// reference exactly:
+//   proximity: TriToTri / TriToBond / BondToBond (dcollid3d.cpp:485-627),
+//              PointToTri (778-922), EdgeToEdge (643-776) with h = thickness
+//   CCD:       MovingTriToTri / MovingTriToBond / MovingBondToBond
+//              (dcollid3d.cpp:203-325) -> MovingPointToTri / MovingEdgeToEdge
+//              (327-369) -> isCoplanar (371-482), h = rounding tolerance
+//   response:  PointToTriImpulse (925-1107), EdgeToEdgeImpulse (1109-1300)
+// Instead of accumulating into per-vertex state, every hit writes a record
+// with its per-point impulse / friction increments; the response stage sums
+// them per vertex in canonical order, reproducing the reference's
+// sequential accumulation bit for bit.
+// In impact-zone passes the points of every sub-test from a pair's first
+// hit onward are merged (the reference's cumulative-status rule,
+// dcollid3d.cpp:222-229, 300-302, 319-321) by a lock-free union-find whose
+// roots are the smallest vertex id of each zone.
+#include "cs_internal.h"
+
+namespace cs {
+
+constexpr int NARROW_THREADS = 128;
+
+struct Sub {
+  double xo[4][3];
+  double v[4][3];
+  bool st[4];
+};
+
+struct Contrib {
+  double imp[4][3];
+  double fric[4][3];
+};
+
+// PointToTriImpulse for static / deformable points (dcollid3d.cpp:925-1107)
+__device__ void d_pt_impulse(const Sub& S, const NarrowParams& P, const double nor[3], double w[3],
+                             double dist, Contrib& C, int* err) {
+  double v_rel[3] = {0.0, 0.0, 0.0}, vn, vt;
+  double impulse = 0.0, m_impulse = 0.0, sum_w = 0.0;
+  const double k = P.k, m = P.m, dt = P.dt, lambda = P.lambda, h = P.thickness;
+  dist = h - dist;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v_rel[i] += S.v[3][i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v_rel[i] -= w[j] * S.v[j][i];
+  }
+  vn = dot3(v_rel, nor);
+  if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
+  else vt = 0.0;
+  if (vn < 0) {
+    if (S.st[3] || (S.st[0] && S.st[1] && S.st[2])) impulse = vn;
+    else impulse = vn * 0.5;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (S.st[i]) w[i] = 0.0;
+      sum_w += w[i];
+    }
+    if (fabs(sum_w) > CS_MACH_EPS) {
+      const double inv = 1.0 / sum_w;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) w[i] = inv * w[i];
+    }
+  }
+  const bool all_rigid = S.st[0] && S.st[1] && S.st[2] && S.st[3];
+  if (vn * dt < 0.1 * dist) {
+    if (!all_rigid) {
+      const double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
+      impulse += tmp;
+    }
+  }
+  if (fabs(sum_w) < CS_MACH_EPS) m_impulse = impulse;
+  else m_impulse = 2.0 * impulse / (1.0 + dot3(w, w));
+  const bool fr = fabs(vt) > CS_ROUND_EPS;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      C.imp[i][j] = w[i] * m_impulse * nor[j];
+      C.fric[i][j] = fr ? smax(-fabs(lambda * w[i] * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    C.imp[3][j] = -(m_impulse * nor[j]);
+    C.fric[3][j] = fr ? smax(-fabs(lambda * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (!S.st[q] && !isfinite(C.imp[q][j])) atomicOr(err, CS_DEVERR_NONFINITE);
+}
+
+// EdgeToEdgeImpulse for static / deformable points (dcollid3d.cpp:1109-1300)
+__device__ void d_ee_impulse(const Sub& S, const NarrowParams& P, const double nor[3], double a,
+                             double b, double dist, Contrib& C, int* err) {
+  double v_rel[3], vn, vt, impulse = 0.0, m_impulse = 0.0;
+  const double k = P.k, m = P.m, dt = P.dt, lambda = P.lambda, h = P.thickness;
+  dist = h - dist;
+  double wa0 = 1.0 - a, wa1 = a, wb0 = 1.0 - b, wb1 = b;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    v_rel[j] = (1.0 - b) * S.v[2][j] + b * S.v[3][j];
+    v_rel[j] -= (1.0 - a) * S.v[0][j] + a * S.v[1][j];
+  }
+  vn = dot3(v_rel, nor);
+  if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
+  else vt = 0.0;
+  if (vn < 0.0) {
+    if ((S.st[0] && S.st[1]) || (S.st[2] && S.st[3])) impulse = vn;
+    else impulse = vn * 0.5;
+    if (S.st[0]) wa0 = 0.0;
+    if (S.st[1]) wa1 = 0.0;
+    if (S.st[2]) wb0 = 0.0;
+    if (S.st[3]) wb1 = 0.0;
+  }
+  const bool all_rigid = S.st[0] && S.st[1] && S.st[2] && S.st[3];
+  if (vn * dt < 0.1 * dist) {
+    if (!all_rigid) {
+      const double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
+      impulse += tmp;
+    }
+  }
+  if (wa0 + wa1 < CS_MACH_EPS || wb0 + wb1 < CS_MACH_EPS) m_impulse = impulse;
+  else m_impulse = 2.0 * impulse / (wa0 * wa0 + wa1 * wa1 + wb0 * wb0 + wb1 * wb1);
+  const bool fr = fabs(vt) > CS_ROUND_EPS;
+  const double wgt[4] = {wa0, wa1, wb0, wb1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double d = wgt[q] * m_impulse * nor[j];
+      C.imp[q][j] = (q < 2) ? d : -d;
+      C.fric[q][j] = fr ? smax(-fabs(lambda * wgt[q] * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (!S.st[q] && !isfinite(C.imp[q][j])) atomicOr(err, CS_DEVERR_NONFINITE);
+}
+
+// PointToTri (dcollid3d.cpp:778-922): c = coords (triangle 0..2, point 3)
+__device__ bool d_point_to_tri(const Sub& S, const double c[4][3], double h, const NarrowParams& P,
+                               Contrib& C, int* err) {
+  double w[3] = {0.0, 0.0, 0.0};
+  double x13[3], x23[3], x43[3], nor[3], nor_mag, dist, det;
+  sub3(c[0], c[2], x13);
+  sub3(c[1], c[2], x23);
+  sub3(c[3], c[2], x43);
+  det = dot3(x13, x13) * dot3(x23, x23) - dot3(x13, x23) * dot3(x13, x23);
+  if (fabs(det) < 1000 * CS_MACH_EPS) return false;
+  cross3(x13, x23, nor);
+  nor_mag = mag3(nor);
+  double x43_old[3];
+  sub3(S.xo[3], S.xo[2], x43_old);
+  dist = dot3(x43_old, nor);
+  const double sgn = (dist >= 0) ? 1.0 : -1.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nor[i] /= nor_mag * sgn;
+  dist = fabs(dot3(x43, nor));
+  w[0] = (dot3(x13, x43) * dot3(x23, x23) - dot3(x23, x43) * dot3(x13, x23)) / det;
+  w[1] = (dot3(x13, x13) * dot3(x23, x43) - dot3(x13, x23) * dot3(x13, x43)) / det;
+  w[2] = 1 - w[0] - w[1];
+  if (fabs(w[0]) < CS_ROUND_EPS || fabs(w[1]) < CS_ROUND_EPS || fabs(w[2]) < CS_ROUND_EPS) {
+    double vec[3] = {S.xo[3][0], S.xo[3][1], S.xo[3][2]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) vec[j] -= w[i] * S.xo[i][j];
+    if (mag3(vec) > CS_ROUND_EPS) {
+      nor[0] = vec[0];
+      nor[1] = vec[1];
+      nor[2] = vec[2];
+    }
+  }
+  nor_mag = mag3(nor);
+  if (nor_mag > CS_ROUND_EPS) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) nor[i] /= nor_mag;
+  } else {
+    atomicOr(err, CS_DEVERR_NONFINITE);
+    return false;
+  }
+  if (dist > h) return false;
+  const double eps = P.round_eps;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (w[i] > 1 + eps || w[i] < -eps) return false;
+  d_pt_impulse(S, P, nor, w, dist, C, err);
+  return true;
+}
+
+// EdgeToEdge (dcollid3d.cpp:643-776): edges c0c1 and c2c3
+__device__ bool d_edge_to_edge(const Sub& S, const double c[4][3], double h, const NarrowParams& P,
+                               Contrib& C, int* err) {
+  double x21[3], x43[3], x31[3], tmp[3], v1[3], v2[3], nor[3], nor_mag, dist, a, b;
+  sub3(c[1], c[0], x21);
+  sub3(c[3], c[2], x43);
+  sub3(c[2], c[0], x31);
+  cross3(x21, x43, tmp);
+  if (mag3(tmp) < CS_ROUND_EPS) return false;
+  a = (dot3(x43, x43) * dot3(x21, x31) - dot3(x21, x43) * dot3(x43, x31)) /
+      (dot3(x21, x21) * dot3(x43, x43) - dot3(x21, x43) * dot3(x21, x43));
+  b = (dot3(x21, x43) * dot3(x21, x31) - dot3(x21, x21) * dot3(x43, x31)) /
+      (dot3(x21, x21) * dot3(x43, x43) - dot3(x21, x43) * dot3(x21, x43));
+  a = smax(smin(a, 1.0), 0.0);
+  b = smax(smin(b, 1.0), 0.0);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v1[i] = a * x21[i];
+    v2[i] = b * x43[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v1[i] = c[0][i] + v1[i];
+    v2[i] = c[2][i] + v2[i];
+  }
+  sub3(v2, v1, nor);
+  nor_mag = mag3(nor);
+  if (nor_mag < 1000 * CS_MACH_EPS) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      nor[j] = (1.0 - b) * S.xo[2][j] + b * S.xo[3][j];
+      nor[j] -= (1.0 - a) * S.xo[0][j] + a * S.xo[1][j];
+    }
+  }
+  dist = dist3(v1, v2);
+  if (dist > h) return false;
+  nor_mag = mag3(nor);
+  if (nor_mag < CS_MACH_EPS) {
+    atomicOr(err, CS_DEVERR_NONFINITE);
+    return false;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) nor[i] /= nor_mag;
+  d_ee_impulse(S, P, nor, a, b, dist, C, err);
+  return true;
+}
+
+// isCoplanar (dcollid3d.cpp:371-482)
+__device__ bool d_is_coplanar(const Sub& S, double dt, double roots[4]) {
+  double v[4][3], x[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v[0][k] = S.v[0][k];
+    x[0][k] = S.xo[0][k];
+  }
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[i][k] = S.v[i][k] - S.v[0][k];
+      x[i][k] = S.xo[i][k] - S.xo[0][k];
+    }
+  double a, b, c, d, vv[3], vx[3], xx[3];
+  vv[0] = v[1][1] * v[2][2] - v[1][2] * v[2][1];
+  vv[1] = v[1][0] * v[2][2] - v[1][2] * v[2][0];
+  vv[2] = v[1][0] * v[2][1] - v[1][1] * v[2][0];
+  vx[0] = v[1][1] * x[2][2] - v[1][2] * x[2][1] - v[2][1] * x[1][2] + v[2][2] * x[1][1];
+  vx[1] = v[1][0] * x[2][2] - v[1][2] * x[2][0] - v[2][0] * x[1][2] + v[2][2] * x[1][0];
+  vx[2] = v[1][0] * x[2][1] - v[1][1] * x[2][0] - v[2][0] * x[1][1] + v[2][1] * x[1][0];
+  xx[0] = x[1][1] * x[2][2] - x[1][2] * x[2][1];
+  xx[1] = x[1][0] * x[2][2] - x[1][2] * x[2][0];
+  xx[2] = x[1][0] * x[2][1] - x[1][1] * x[2][0];
+  a = v[3][0] * vv[0] - v[3][1] * vv[1] + v[3][2] * vv[2];
+  b = x[3][0] * vv[0] - x[3][1] * vv[1] + x[3][2] * vv[2] + v[3][0] * vx[0] - v[3][1] * vx[1] +
+      v[3][2] * vx[2];
+  c = x[3][0] * vx[0] - x[3][1] * vx[1] + x[3][2] * vx[2] + v[3][0] * xx[0] - v[3][1] * xx[1] +
+      v[3][2] * xx[2];
+  d = x[3][0] * xx[0] - x[3][1] * xx[1] + x[3][2] * xx[2];
+  roots[0] = roots[1] = roots[2] = -1.0;
+  roots[3] = dt;
+  if (fabs(a) > CS_MACH_EPS) {
+    b /= a;
+    c /= a;
+    d /= a;
+    a = b;
+    b = c;
+    c = d;
+    const double Q = (a * a - 3 * b) / 9;
+    const double R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+    const double Q3 = Q * Q * Q, R2 = R * R;
+    if (R2 < Q3) {
+      const double Qsqrt = sqrt(Q);
+      const double theta = cs_det_acos(R / sqrt(Q3));
+      roots[0] = -2 * Qsqrt * cs_det_cos(theta / 3) - a / 3;
+      roots[1] = -2 * Qsqrt * cs_det_cos((theta + 2 * CS_PI) / 3) - a / 3;
+      roots[2] = -2 * Qsqrt * cs_det_cos((theta - 2 * CS_PI) / 3) - a / 3;
+    } else {
+      const double sgn = (R > 0) ? 1.0 : -1.0;
+      const double A = -sgn * cs_det_cbrt(fabs(R) + sqrt(R2 - Q3));
+      const double B = (fabs(A) < CS_ROUND_EPS) ? 0.0 : Q / A;
+      roots[0] = (A + B) - a / 3.0;
+      if (fabs(A - B) < CS_ROUND_EPS) roots[1] = roots[2] = -0.5 * (A + B) - a / 3.0;
+    }
+  } else {
+    a = b;
+    b = c;
+    c = d;
+    const double delta = b * b - 4.0 * a * c;
+    if (fabs(a) > CS_ROUND_EPS && delta > 0) {
+      const double ds = sqrt(delta);
+      roots[0] = (-b + ds) / (2.0 * a);
+      roots[1] = (-b - ds) / (2.0 * a);
+    } else if (fabs(a) < CS_ROUND_EPS && fabs(b) > CS_ROUND_EPS) {
+      roots[0] = -c / b;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    roots[i] = roots[i] - CS_MACH_EPS;
+    if (roots[i] < 0 || roots[i] > dt) roots[i] = -1;
+  }
+  double t;
+  if (roots[0] > roots[1]) { t = roots[0]; roots[0] = roots[1]; roots[1] = t; }
+  if (roots[0] > roots[2]) { t = roots[0]; roots[0] = roots[2]; roots[2] = t; }
+  if (roots[1] > roots[2]) { t = roots[1]; roots[1] = roots[2]; roots[2] = t; }
+  return roots[0] > CS_MACH_EPS || roots[1] > CS_MACH_EPS || roots[2] > CS_MACH_EPS;
+}
+
+__device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contrib& C, int* err) {
+  double c[4][3];
+  if (!P.ccd) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[q][k] = S.xo[q][k];
+    return is_pt ? d_point_to_tri(S, c, P.thickness, P, C, err)
+                 : d_edge_to_edge(S, c, P.thickness, P, C, err);
+  }
+  double roots[4];
+  if (!d_is_coplanar(S, P.dt, roots)) return false;
+  for (int r = 0; r < 4; ++r) {
+    if (roots[r] < 0) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[q][k] = S.xo[q][k] + roots[r] * S.v[q][k];
+    const bool hit = is_pt ? d_point_to_tri(S, c, P.round_eps, P, C, err)
+                           : d_edge_to_edge(S, c, P.round_eps, P, C, err);
+    if (hit) return true;
+  }
+  return false;
+}
+
+// local point indices of sub-test s.  kind: 0 tri-tri (A 0..2, B 3..5),
+// 1 tri-bond (tri 0..2, bond 3..4), 2 bond-bond (A 0..1, B 2..3)
+__device__ inline void subtest_points(int kind, int s, int ccd, int q[4], bool* is_pt) {
+  if (kind == 0) {
+    if (s < 6) {
+      const int k = s / 3, i = s - 3 * k;
+      // proximity: k=0 triangle B, point A[i]; CCD: k=0 triangle A, point B[i]
+      const bool tri_is_a = (k == 0) == (ccd != 0);
+      const int tb = tri_is_a ? 0 : 3, pb = tri_is_a ? 3 : 0;
+      q[0] = tb; q[1] = tb + 1; q[2] = tb + 2; q[3] = pb + i;
+      *is_pt = true;
+    } else {
+      const int i = (s - 6) / 3, j = (s - 6) - 3 * i;
+      q[0] = i; q[1] = (i + 1) % 3; q[2] = 3 + j; q[3] = 3 + (j + 1) % 3;
+      *is_pt = false;
+    }
+  } else if (kind == 1) {
+    if (s < 2) {
+      q[0] = 0; q[1] = 1; q[2] = 2; q[3] = 3 + s;
+      *is_pt = true;
+    } else {
+      const int i = s - 2;
+      q[0] = i; q[1] = (i + 1) % 3; q[2] = 3; q[3] = 4;
+      *is_pt = false;
+    }
+  } else {
+    q[0] = 0; q[1] = 1; q[2] = 2; q[3] = 3;
+    *is_pt = false;
+  }
+}
+
+__device__ inline int uf_load(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ int uf_find(int* uf, int x) {
+  for (;;) {
+    const int p = uf_load(&uf[x]);
+    if (p == x) return x;
+    const int gp = uf_load(&uf[p]);
+    if (gp == p) return p;
+    atomicCAS(&uf[x], p, gp);  // path halving; parents only decrease
+    x = gp;
+  }
+}
+__device__ void uf_union(int* uf, int a, int b) {
+  for (;;) {
+    a = uf_find(uf, a);
+    b = uf_find(uf, b);
+    if (a == b) return;
+    if (a < b) { const int t = a; a = b; b = t; }
+    if (atomicCAS(&uf[a], a, b) == a) return;  // larger root under smaller
+  }
+}
+
+__global__ void __launch_bounds__(NARROW_THREADS)
+    k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
+             const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
+             const uint8_t* __restrict__ fixed, NarrowParams P, int* __restrict__ uf,
+             hit_rec* __restrict__ hits, int hit_cap, int* __restrict__ counters,
+             int4* __restrict__ plog, int plog_cap, int* __restrict__ err) {
+  __shared__ double lx[6 * 3 * NARROW_THREADS];
+  __shared__ double lv[6 * 3 * NARROW_THREADS];
+  __shared__ uint8_t lst[6 * NARROW_THREADS];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * NARROW_THREADS + t;
+  if (i >= npairs) return;
+  const int2 pr = pairs[i];
+  const int A = pr.y, B = pr.x;  // canonical orientation: a = hi, b = lo
+  const int4 ta = ev[A], tb = ev[B];
+  const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
+  int kind, nloc, nsub;
+  int g[6];
+  if (tri_a && tri_b) {
+    kind = 0; nloc = 6; nsub = 15;
+    g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
+  } else if (tri_a || tri_b) {
+    const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
+    kind = 1; nloc = 5; nsub = 5;
+    g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
+  } else {
+    kind = 2; nloc = 4; nsub = 1;
+    g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
+  }
+  for (int k = 0; k < nloc; ++k) {
+    const dvec4 xo = x_old[g[k]];
+    const dvec4 vv = av[g[k]];
+    lx[(k * 3 + 0) * NARROW_THREADS + t] = xo.x;
+    lx[(k * 3 + 1) * NARROW_THREADS + t] = xo.y;
+    lx[(k * 3 + 2) * NARROW_THREADS + t] = xo.z;
+    lv[(k * 3 + 0) * NARROW_THREADS + t] = vv.x;
+    lv[(k * 3 + 1) * NARROW_THREADS + t] = vv.y;
+    lv[(k * 3 + 2) * NARROW_THREADS + t] = vv.z;
+    lst[k * NARROW_THREADS + t] = fixed[g[k]];
+  }
+  int mask = 0, first_hit = -1;
+  for (int s = 0; s < nsub; ++s) {
+    int q[4];
+    bool is_pt;
+    subtest_points(kind, s, P.ccd, q, &is_pt);
+    Sub S;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        S.xo[p][k] = lx[(q[p] * 3 + k) * NARROW_THREADS + t];
+        S.v[p][k] = lv[(q[p] * 3 + k) * NARROW_THREADS + t];
+      }
+      S.st[p] = lst[q[p] * NARROW_THREADS + t] != 0;
+    }
+    Contrib C;
+    if (d_subtest(S, is_pt, P, C, err)) {
+      mask |= 1 << s;
+      if (first_hit < 0) first_hit = s;
+      const int slot = atomicAdd(&counters[CNT_HITS], 1);
+      if (slot < hit_cap) {
+        hit_rec& h = hits[slot];
+        h.key = hit_key((uint32_t)B, (uint32_t)A, (uint32_t)s);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          h.v[p] = S.st[p] ? -1 : g[q[p]];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            h.imp[p][k] = C.imp[p][k];
+            h.fric[p][k] = C.fric[p][k];
+          }
+        }
+      }
+    }
+  }
+  if (mask) {
+    atomicAdd(&counters[CNT_HIT_PAIRS], 1);
+    atomicAdd(&counters[CNT_HIT_SUB], __popc(mask));
+    if (P.log_pairs) {
+      const int slot = atomicAdd(&counters[CNT_LOG], 1);
+      if (slot < plog_cap) plog[slot] = make_int4(P.pass_id, B, A, mask);
+    }
+    if (P.zone_mode) {
+      for (int s = first_hit; s < nsub; ++s) {
+        int q[4];
+        bool is_pt;
+        subtest_points(kind, s, P.ccd, q, &is_pt);
+        // createImpZone(pts, 4): merge every pair of the four points
+        uf_union(uf, g[q[0]], g[q[1]]);
+        uf_union(uf, g[q[0]], g[q[2]]);
+        uf_union(uf, g[q[0]], g[q[3]]);
+      }
+    }
+  }
+}
+
+void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, int npairs) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_narrow, dim3((npairs + NARROW_THREADS - 1) / NARROW_THREADS),
+                     dim3(NARROW_THREADS), 0, st, S.pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
+                     S.uf, S.hits, S.hit_cap, S.counters, S.plog, S.plog_cap, S.err);
+}
+
+}  // namespace cs

@@ -1,0 +1,517 @@
+// Response and per-vertex stages for gfx950.
+//
+//  * apply_hits: hit records are sorted by their canonical key (pair (lo,hi),
+//    sub-test), expanded to one record per deformable point in that order,
+//    stably sorted by vertex, and each vertex segment is summed left to right
+//    from 0.0 by one lane -- the same sequence of additions the reference
+//    performs into collsnImpulse / friction (dcollid3d.cpp:1053-1079,
+//    1235-1284).  The Jacobi update avgVel += (imp + fric) / count is
+//    updateAverageVelocity (dcollid.cpp:677-751).  No atomics touch values,
+//    so results are deterministic.
+//  * update_zones: union-find roots (smallest vertex id of each zone) ->
+//    stable sort of vertices by root -> one lane per zone computes the rigid
+//    motion of updateImpactListVelocity (dcollid3d.cpp:70-200) over its
+//    members in ascending vertex id.
+//  * strain_limit: reduceSuperelast (dcollid.cpp:486-596) is an in-place
+//    Gauss-Seidel sweep in element order.  Elements are bucketed into levels
+//    (level = 1 + max level of earlier elements sharing a vertex); elements of
+//    one level share no vertex and each vertex sees its elements in the
+//    original order, so a level-by-level sweep reproduces the sequential
+//    one exactly.  A parallel pre-check skips sweeps in which no edge is over
+//    the limit (such a sweep changes nothing and ends the loop).
+#include "cs_internal.h"
+
+namespace cs {
+
+__global__ void k_predict(dvec4* __restrict__ x, dvec4* __restrict__ x_old,
+                          const dvec4* __restrict__ vel, double dt, int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const dvec4 p = x[v], u = vel[v];
+  dvec4 o = p;
+  dvec4 n;
+  n.x = o.x + dt * u.x;
+  n.y = o.y + dt * u.y;
+  n.z = o.z + dt * u.z;
+  n.w = 0.0;
+  x_old[v] = o;
+  x[v] = n;
+}
+
+// computeAverageVelocity (dcollid.cpp:160-220)
+__global__ void k_average_velocity(dvec4* __restrict__ x, const dvec4* __restrict__ x_old,
+                                   dvec4* __restrict__ av, double dt, int nv,
+                                   int* __restrict__ err) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const dvec4 p = x[v], o = x_old[v];
+  dvec4 a;
+  if (dt > CS_ROUND_EPS) {
+    a.x = (p.x - o.x) / dt;
+    a.y = (p.y - o.y) / dt;
+    a.z = (p.z - o.z) / dt;
+  } else {
+    a.x = a.y = a.z = 0.0;
+  }
+  a.w = 0.0;
+  if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(err, CS_DEVERR_NONFINITE);
+  av[v] = a;
+  x[v] = o;
+}
+
+__global__ void k_uf_init(int32_t* __restrict__ uf, int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) uf[v] = v;
+}
+
+__global__ void k_hit_keys(const hit_rec* __restrict__ hits, int n, uint64_t* __restrict__ keys,
+                           uint32_t* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = hits[i].key;
+  idx[i] = (uint32_t)i;
+}
+
+// one record per (hit in canonical order, point); static points get the
+// sentinel key nv and sort last
+__global__ void k_expand_records(const hit_rec* __restrict__ hits,
+                                 const uint32_t* __restrict__ sorted_idx, int n, int nv,
+                                 uint32_t* __restrict__ rkey, uint32_t* __restrict__ rval) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 4 * n) return;
+  const int r = i >> 2, p = i & 3;
+  const uint32_t h = sorted_idx[r];
+  const int v = hits[h].v[p];
+  rkey[i] = (v >= 0) ? (uint32_t)v : (uint32_t)nv;
+  rval[i] = h * 4u + (uint32_t)p;
+}
+
+__global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __restrict__ rval,
+                        int nrec, int nv, const hit_rec* __restrict__ hits,
+                        dvec4* __restrict__ av, uint8_t* __restrict__ has,
+                        int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrec) return;
+  const uint32_t v = rkey[i];
+  if (v >= (uint32_t)nv) return;
+  if (i > 0 && rkey[i - 1] == v) return;  // not the head of the vertex segment
+  double imp[3] = {0.0, 0.0, 0.0}, fric[3] = {0.0, 0.0, 0.0};
+  int count = 0;
+  for (int j = i; j < nrec && rkey[j] == v; ++j) {
+    const uint32_t hv = rval[j];
+    const hit_rec& h = hits[hv >> 2];
+    const int p = hv & 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      imp[k] += h.imp[p][k];
+      fric[k] += h.fric[p][k];
+    }
+    ++count;
+  }
+  dvec4 a = av[v];
+  a.x += (imp[0] + fric[0]) / count;
+  a.y += (imp[1] + fric[1]) / count;
+  a.z += (imp[2] + fric[2]) / count;
+  if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(err, CS_DEVERR_NONFINITE);
+  av[v] = a;
+  has[v] = 1;
+}
+
+static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rounded to 8
+  int b = 0;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  b = (b + 7) & ~7;
+  return b < 8 ? 8 : b;
+}
+
+hipError_t apply_hits(hipStream_t st, DevState& S, int nhits) {
+  if (nhits <= 0) return hipSuccess;
+  const int nb = (nhits + 255) / 256;
+  hipLaunchKernelGGL(k_hit_keys, dim3(nb), dim3(256), 0, st, S.hits, nhits, S.hkey, S.hidx);
+  bool in_tmp = false;
+  const int kbits = bits_for(hit_key((uint32_t)S.ne, (uint32_t)S.ne, 255u));
+  CS_HIP_RET(radix_sort_u64(st, S.hkey, S.hidx, S.hkey_tmp, S.hidx_tmp, S.sort_hist, nhits, kbits,
+                            &in_tmp));
+  const uint32_t* sidx = in_tmp ? S.hidx_tmp : S.hidx;
+  const int nrec = 4 * nhits;
+  hipLaunchKernelGGL(k_expand_records, dim3((nrec + 255) / 256), dim3(256), 0, st, S.hits, sidx,
+                     nhits, S.nv, S.rkey, S.rval);
+  CS_HIP_RET(radix_sort_u32(st, S.rkey, S.rval, S.rkey_tmp, S.rval_tmp, S.sort_hist, nrec,
+                            bits_for((uint64_t)S.nv), &in_tmp));
+  const uint32_t* rk = in_tmp ? S.rkey_tmp : S.rkey;
+  const uint32_t* rv = in_tmp ? S.rval_tmp : S.rval;
+  hipLaunchKernelGGL(k_apply, dim3((nrec + 255) / 256), dim3(256), 0, st, rk, rv, nrec, S.nv,
+                     S.hits, S.av, S.has, S.err);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// impact zones
+__device__ inline int uf_root(const int32_t* uf, int x) {
+  int p = uf[x];
+  while (p != x) {
+    x = p;
+    p = uf[x];
+  }
+  return x;
+}
+
+__global__ void k_zone_roots(const int32_t* __restrict__ uf, int nv, int32_t* __restrict__ zsize,
+                             uint32_t* __restrict__ root_of) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int r = uf_root(uf, v);
+  root_of[v] = (uint32_t)r;
+  atomicAdd(&zsize[r], 1);
+}
+
+__global__ void k_zone_keys(const uint32_t* __restrict__ root_of, const int32_t* __restrict__ zsize,
+                            int nv, uint32_t* __restrict__ zkey, uint32_t* __restrict__ zval) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const uint32_t r = root_of[v];
+  zkey[v] = (zsize[r] >= 2) ? r : (uint32_t)nv;
+  zval[v] = (uint32_t)v;
+}
+
+// updateImpactListVelocity (dcollid3d.cpp:70-200), members ascending
+__global__ void k_zone_update(const uint32_t* __restrict__ zkey, const uint32_t* __restrict__ zval,
+                              int nv, const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
+                              const uint8_t* __restrict__ fixed, double m, double dt,
+                              int* __restrict__ counters, int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const uint32_t z = zkey[i];
+  if (z >= (uint32_t)nv) return;
+  if (i > 0 && zkey[i - 1] == z) return;
+  int end = i;
+  while (end < nv && zkey[end] == z) ++end;
+  atomicAdd(&counters[CNT_ZONES], 1);
+  double x_cm[3] = {0, 0, 0}, v_cm[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+  double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tmp[3][3];
+  int num_pts = 0;
+  for (int j = i; j < end; ++j) {
+    const int p = zval[j];
+    const dvec4 o = x_old[p], a = av[p];
+    num_pts++;
+    x_cm[0] += o.x; x_cm[1] += o.y; x_cm[2] += o.z;
+    v_cm[0] += a.x; v_cm[1] += a.y; v_cm[2] += a.z;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x_cm[k] /= num_pts;
+    v_cm[k] /= num_pts;
+  }
+  for (int j = i; j < end; ++j) {
+    const int p = zval[j];
+    const dvec4 o = x_old[p], a = av[p];
+    const double xo[3] = {o.x, o.y, o.z}, vv[3] = {a.x, a.y, a.z};
+    double dx[3], dv[3], Li[3];
+    sub3(xo, x_cm, dx);
+    sub3(vv, v_cm, dv);
+    cross3(dx, dv, Li);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Li[k] = m * Li[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L[k] = Li[k] + L[k];
+  }
+  for (int j = i; j < end; ++j) {
+    const int p = zval[j];
+    const dvec4 o = x_old[p];
+    const double xo[3] = {o.x, o.y, o.z};
+    double dx[3];
+    sub3(xo, x_cm, dx);
+    const double mag_dx = mag3(dx);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        tmp[r][c] = -dx[r] * dx[c];
+        if (r == c) tmp[r][c] += mag_dx * mag_dx;
+        I[r][c] += tmp[r][c] * m;
+      }
+  }
+  double w[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) tmp[r][q] = I[r][q];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) tmp[r][c] = L[r];
+    if (det3(I) < CS_ROUND_EPS) w[c] = 0.0;
+    else w[c] = det3(tmp) / det3(I);
+  }
+  const double mag_w = mag3(w);
+  for (int j = i; j < end; ++j) {
+    const int p = zval[j];
+    if (fixed[p]) continue;
+    const dvec4 o = x_old[p];
+    const double xo[3] = {o.x, o.y, o.z};
+    double dx[3], xF[3], xR[3], wxR[3], tmpV[3], x_new[3];
+    sub3(xo, x_cm, dx);
+    if (mag_w < CS_ROUND_EPS) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        xF[k] = dx[k];
+        wxR[k] = 0.0;
+      }
+      sub3(dx, xF, xR);
+    } else {
+      const double sc = dot3(dx, w) / dot3(w, w);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xF[k] = sc * w[k];
+      sub3(dx, xF, xR);
+      const double sw = cs_det_sin(dt * mag_w) / mag_w;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) tmpV[k] = sw * w[k];
+      cross3(tmpV, xR, wxR);
+    }
+    dvec4 a;
+    double an[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x_new[k] = x_cm[k] + dt * v_cm[k] + xF[k] + cs_det_cos(dt * mag_w) * xR[k] + wxR[k];
+      an[k] = (x_new[k] - xo[k]) / dt;
+    }
+    if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
+    a.x = an[0]; a.y = an[1]; a.z = an[2]; a.w = 0.0;
+    av[p] = a;
+  }
+}
+
+hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt) {
+  const int nv = S.nv;
+  const int nb = (nv + 255) / 256;
+  CS_HIP_RET(hipMemsetAsync(S.zsize, 0, sizeof(int32_t) * (size_t)nv, st));
+  hipLaunchKernelGGL(k_zone_roots, dim3(nb), dim3(256), 0, st, S.uf, nv, S.zsize, S.zval_tmp);
+  hipLaunchKernelGGL(k_zone_keys, dim3(nb), dim3(256), 0, st, S.zval_tmp, S.zsize, nv, S.zkey,
+                     S.zval);
+  bool in_tmp = false;
+  CS_HIP_RET(radix_sort_u32(st, S.zkey, S.zval, S.zkey_tmp, S.zval_tmp, S.sort_hist, nv,
+                            bits_for((uint64_t)nv), &in_tmp));
+  const uint32_t* zk = in_tmp ? S.zkey_tmp : S.zkey;
+  const uint32_t* zv = in_tmp ? S.zval_tmp : S.zval;
+  hipLaunchKernelGGL(k_zone_update, dim3(nb), dim3(256), 0, st, zk, zv, nv, S.x_old, S.av, S.fixed,
+                     m, dt, S.counters, S.err);
+  return hipGetLastError();
+}
+
+__global__ void k_zone_labels(const int32_t* __restrict__ uf, int nv, int32_t* __restrict__ out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nv) out[v] = uf_root(uf, v);
+}
+
+void launch_zone_labels(hipStream_t st, DevState& S, int32_t* out) {
+  hipLaunchKernelGGL(k_zone_labels, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.uf, S.nv, out);
+}
+
+// ---------------------------------------------------------------------------
+// detectDomainBoundaryCollision (dcollid.cpp:116-158); a point's second visit
+// is a no-op, so each point is handled once
+__global__ void k_domain(dvec4* __restrict__ av, dvec4* __restrict__ x,
+                         const dvec4* __restrict__ x_old, uint8_t* __restrict__ has, int nv,
+                         double lo0, double lo1, double lo2, double hi0, double hi1, double hi2,
+                         double dt, double mu, double thickness) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  dvec4 a4 = av[v], o4 = x_old[v], c4 = x[v];
+  double a[3] = {a4.x, a4.y, a4.z}, o[3] = {o4.x, o4.y, o4.z}, c[3] = {c4.x, c4.y, c4.z};
+  const double L[3] = {lo0, lo1, lo2}, U[3] = {hi0, hi1, hi2};
+  double dv = 0;
+  bool touched = false;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double cand = o[j] + dt * a[j];
+    if (cand <= L[j]) {
+      touched = true;
+      dv = fabs(a[j]);
+      a[j] = 0.0;
+      c[j] = L[j] + thickness;
+    } else if (cand >= U[j]) {
+      touched = true;
+      dv = fabs(a[j]);
+      a[j] = 0.0;
+      c[j] = U[j] - thickness;
+    }
+  }
+  const double preVt = mag3(a);
+  if (preVt > CS_MACH_EPS) {
+    const double f = smax(1.0 - mu * dv / preVt, 0.0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) a[j] *= f;
+  }
+  if (touched) has[v] = 1;
+  av[v] = dvec4{a[0], a[1], a[2], 0.0};
+  x[v] = dvec4{c[0], c[1], c[2], 0.0};
+}
+
+void launch_domain_boundary(hipStream_t st, DevState& S, const double lo[3], const double hi[3],
+                            double dt, double mu, double thickness) {
+  hipLaunchKernelGGL(k_domain, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.av, S.x, S.x_old,
+                     S.has, S.nv, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], dt, mu, thickness);
+}
+
+// updateFinalPosition (dcollid.cpp:562-584)
+__global__ void k_final_position(dvec4* __restrict__ x, const dvec4* __restrict__ x_old,
+                                 const dvec4* __restrict__ av, double dt, int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const dvec4 o = x_old[v], a = av[v];
+  x[v] = dvec4{o.x + a.x * dt, o.y + a.y * dt, o.z + a.z * dt, 0.0};
+}
+
+void launch_final_position(hipStream_t st, DevState& S, double dt) {
+  hipLaunchKernelGGL(k_final_position, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old,
+                     S.av, dt, S.nv);
+}
+
+// updateFinalVelocity (dcollid.cpp:598-624)
+__global__ void k_final_velocity(dvec4* __restrict__ vel, const dvec4* __restrict__ av,
+                                 const uint8_t* __restrict__ has, int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  if (has[v]) {
+    const dvec4 a = av[v];
+    vel[v] = dvec4{a.x, a.y, a.z, 0.0};
+  }
+}
+
+void launch_final_velocity(hipStream_t st, DevState& S) {
+  hipLaunchKernelGGL(k_final_velocity, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.vel, S.av,
+                     S.has, S.nv);
+}
+
+void launch_predict(hipStream_t st, DevState& S, double dt) {
+  hipLaunchKernelGGL(k_predict, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old, S.vel,
+                     dt, S.nv);
+}
+
+void launch_average_velocity(hipStream_t st, DevState& S, double dt) {
+  hipLaunchKernelGGL(k_average_velocity, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old,
+                     S.av, dt, S.nv, S.err);
+}
+
+void launch_uf_init(hipStream_t st, DevState& S) {
+  hipLaunchKernelGGL(k_uf_init, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.uf, S.nv);
+}
+
+// ---------------------------------------------------------------------------
+// strain limiting
+__device__ inline bool edge_over_limit(const dvec4& o0, const dvec4& o1, const dvec4& a0,
+                                       const dvec4& a1, double len0, double dt, bool* degenerate) {
+  const double xc0[3] = {o0.x + dt * a0.x, o0.y + dt * a0.y, o0.z + dt * a0.z};
+  const double xc1[3] = {o1.x + dt * a1.x, o1.y + dt * a1.y, o1.z + dt * a1.z};
+  const double p0[3] = {o0.x, o0.y, o0.z}, p1[3] = {o1.x, o1.y, o1.z};
+  const double len_new = dist3(xc0, xc1);
+  const double len_old = dist3(p0, p1);
+  *degenerate = false;
+  if (len_old > CS_ROUND_EPS && len_new > CS_ROUND_EPS) {
+    const double strain_rate = (len_new - len_old) / len_old;
+    const double strain = (len_new - len0) / len0;
+    return fabs(strain) > CS_SUPERELAS_TOL || fabs(strain_rate) > CS_SUPERELAS_TOL;
+  }
+  *degenerate = true;
+  return true;
+}
+
+// parallel check: does any edge exceed the limit with the current velocities?
+__global__ void k_strain_check(const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
+                               const dvec4* __restrict__ len0, const dvec4* __restrict__ x_old,
+                               const dvec4* __restrict__ av, double dt, int ne,
+                               int* __restrict__ counters) {
+  if (counters[CNT_STRAIN_FLAG] < 0) return;  // loop already finished
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  bool any = false;
+  if (e < ne && !(eflags[e] & EF_RIGID)) {
+    const int4 t = ev[e];
+    const int np = (t.z >= 0) ? 3 : 2;
+    const int vid[3] = {t.x, t.y, t.z};
+    const dvec4 l0 = len0[e];
+    const double L0[3] = {l0.x, l0.y, l0.z};
+    for (int j = 0; j < ((np == 2) ? 1 : np); ++j) {
+      const int p0 = vid[j % np], p1 = vid[(j + 1) % np];
+      bool deg;
+      if (edge_over_limit(x_old[p0], x_old[p1], av[p0], av[p1], L0[j], dt, &deg)) any = true;
+    }
+  }
+  if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(&counters[CNT_STRAIN_FLAG], 1);
+}
+
+// one exact sequential sweep, level by level, by a single work-group
+constexpr int STRAIN_THREADS = 1024;
+__global__ void __launch_bounds__(STRAIN_THREADS)
+    k_strain_sweep(const int32_t* __restrict__ lvl_ptr, const int32_t* __restrict__ lvl_elems,
+                   int n_levels, const int4* __restrict__ ev, const dvec4* __restrict__ len0,
+                   const dvec4* __restrict__ x_old, dvec4* av, double dt,
+                   int* __restrict__ counters) {
+  __shared__ int s_flag, s_edges, s_degen, s_state;
+  if (threadIdx.x == 0) {
+    s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweep needed
+    s_flag = 0;
+    s_edges = 0;
+    s_degen = 0;
+  }
+  __syncthreads();
+  const int state = s_state;
+  if (state < 0) return;
+  if (state == 0) {  // nothing over the limit: this sweep is a no-op and ends the loop
+    if (threadIdx.x == 0) {
+      counters[CNT_STRAIN_FLAG] = -1;
+      counters[CNT_STRAIN_EDGES] = 0;
+      atomicAdd(&counters[CNT_STRAIN_DEGEN + 1], 1);  // sweeps executed
+    }
+    return;
+  }
+  for (int L = 0; L < n_levels; ++L) {
+    const int b = lvl_ptr[L], e_end = lvl_ptr[L + 1];
+    for (int idx = b + threadIdx.x; idx < e_end; idx += STRAIN_THREADS) {
+      const int e = lvl_elems[idx];
+      const int4 t = ev[e];
+      const int np = (t.z >= 0) ? 3 : 2;
+      const int vid[3] = {t.x, t.y, t.z};
+      const dvec4 l0 = len0[e];
+      const double L0[3] = {l0.x, l0.y, l0.z};
+      for (int j = 0; j < ((np == 2) ? 1 : np); ++j) {
+        const int p0 = vid[j % np], p1 = vid[(j + 1) % np];
+        const dvec4 a0 = av[p0], a1 = av[p1];
+        bool deg;
+        if (edge_over_limit(x_old[p0], x_old[p1], a0, a1, L0[j], dt, &deg)) {
+          double vt[3] = {a0.x + a1.x, a0.y + a1.y, a0.z + a1.z};
+          vt[0] = 0.5 * vt[0];
+          vt[1] = 0.5 * vt[1];
+          vt[2] = 0.5 * vt[2];
+          const dvec4 nv4 = dvec4{vt[0], vt[1], vt[2], 0.0};
+          av[p0] = nv4;
+          av[p1] = nv4;
+          s_flag = 1;
+          atomicAdd(&s_edges, 1);
+          if (deg) atomicAdd(&s_degen, 1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counters[CNT_STRAIN_EDGES] = s_edges;
+    counters[CNT_STRAIN_DEGEN] += s_degen;
+    counters[CNT_STRAIN_DEGEN + 1] += 1;  // sweeps executed
+    counters[CNT_STRAIN_FLAG] = s_flag ? 0 : -1;  // next check decides; no edge -> finished
+  }
+}
+
+hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, int* sweeps_out) {
+  (void)sweeps_out;
+  const int ne = S.ne;
+  if (ne == 0 || S.n_sched == 0 || max_sweeps <= 0) return hipSuccess;
+  for (int s = 0; s < max_sweeps; ++s) {
+    hipLaunchKernelGGL(k_strain_check, dim3((ne + 255) / 256), dim3(256), 0, st, S.ev, S.eflags,
+                       S.len0, S.x_old, S.av, dt, ne, S.counters);
+    hipLaunchKernelGGL(k_strain_sweep, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
+                       S.lvl_elems, S.n_levels, S.ev, S.len0, S.x_old, S.av, dt, S.counters);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace cs

@@ -1,0 +1,178 @@
+// Stable LSD radix sort of (key, value) pairs for gfx950.
+//
+// One pass per 8-bit digit, three kernels per pass (reduce-then-scan):
+//   1. k_digit_hist  : per-tile 256-bin histogram (LDS atomics), written
+//                      digit-major so a plain exclusive scan gives offsets.
+//   2. k_scan_single : one work-group exclusive scan of the histogram table.
+//   3. k_scatter     : per-tile stable ranking.  Each 64-lane wavefront owns a
+//                      contiguous quarter of the tile and walks it in 64-key
+//                      rounds; lanes holding the same digit are found with 8
+//                      ballots (one per digit bit), their rank is the popcount
+//                      of lower matching lanes, and a per-wave LDS counter
+//                      carries counts across rounds.  Keys keep their input
+//                      order within a digit, so the sort is stable.
+#include "cs_internal.h"
+
+namespace cs {
+
+constexpr int SORT_THREADS = 256;  // 4 wavefronts
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr int SORT_ROUNDS = 8;     // keys per lane
+constexpr int SORT_TILE = SORT_THREADS * SORT_ROUNDS;  // 2048
+constexpr int SORT_SUB = SORT_TILE / SORT_WAVES;       // 512 keys per wave
+
+template <typename K>
+__global__ void __launch_bounds__(SORT_THREADS) k_digit_hist(const K* __restrict__ keys, int n,
+                                                             int shift, uint32_t* __restrict__ hist,
+                                                             int nblocks) {
+  __shared__ uint32_t h[256];
+  for (int i = threadIdx.x; i < 256; i += SORT_THREADS) h[i] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * SORT_TILE;
+  for (int k = threadIdx.x; k < SORT_TILE; k += SORT_THREADS) {
+    int idx = base + k;
+    if (idx < n) {
+      uint32_t d = (uint32_t)((keys[idx] >> shift) & 255u);
+      atomicAdd(&h[d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 256; i += SORT_THREADS) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
+}
+
+// Exclusive scan of n uint32 values in place by one work-group of 1024.
+__global__ void __launch_bounds__(1024) k_scan_single(uint32_t* __restrict__ data, int n) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b = t * chunk;
+  const int e = min(n, b + chunk);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += data[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = (t >= off) ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;
+  for (int i = b; i < e; ++i) {
+    uint32_t v = data[i];
+    data[i] = run;
+    run += v;
+  }
+}
+
+template <typename K>
+__global__ void __launch_bounds__(SORT_THREADS) k_scatter(const K* __restrict__ keys_in,
+                                                          const uint32_t* __restrict__ vals_in,
+                                                          K* __restrict__ keys_out,
+                                                          uint32_t* __restrict__ vals_out, int n,
+                                                          int shift,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          int nblocks) {
+  __shared__ uint32_t cnt[SORT_WAVES][256];
+  __shared__ uint32_t boff[256];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < SORT_WAVES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+  for (int i = threadIdx.x; i < 256; i += SORT_THREADS)
+    boff[i] = offsets[(size_t)i * nblocks + blockIdx.x];
+  __syncthreads();
+
+  const int base = blockIdx.x * SORT_TILE + w * SORT_SUB;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  K key[SORT_ROUNDS];
+  uint32_t val[SORT_ROUNDS];
+  uint32_t rank[SORT_ROUNDS];
+  uint32_t dig[SORT_ROUNDS];
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    const int idx = base + r * 64 + lane;
+    const bool valid = idx < n;
+    K k = valid ? keys_in[idx] : (K)0;
+    uint32_t d = (uint32_t)((k >> shift) & 255u);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t before = cnt[w][d];
+    if (valid) {
+      const int leader = __ffsll((unsigned long long)peers) - 1;
+      if (lane == leader) cnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    rank[r] = before + (uint32_t)__popcll(peers & lt_mask);
+    key[r] = k;
+    val[r] = valid ? vals_in[idx] : 0u;
+    dig[r] = valid ? d : 256u;
+  }
+  __syncthreads();
+  // exclusive prefix of the per-wave digit counts across the block's waves
+  for (int d = threadIdx.x; d < 256; d += SORT_THREADS) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < SORT_WAVES; ++ww) {
+      uint32_t c = cnt[ww][d];
+      cnt[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ROUNDS; ++r) {
+    if (dig[r] < 256u) {
+      const uint32_t pos = boff[dig[r]] + cnt[w][dig[r]] + rank[r];
+      keys_out[pos] = key[r];
+      vals_out[pos] = val[r];
+    }
+  }
+}
+
+template <typename K>
+static hipError_t radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, K* tmp_keys,
+                                  uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
+                                  bool* result_in_tmp) {
+  *result_in_tmp = false;
+  if (n <= 1 || bits <= 0) return hipSuccess;
+  const int nblocks = (n + SORT_TILE - 1) / SORT_TILE;
+  K* kin = keys;
+  uint32_t* vin = vals;
+  K* kout = tmp_keys;
+  uint32_t* vout = tmp_vals;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_digit_hist<K>, dim3(nblocks), dim3(SORT_THREADS), 0, st, kin, n, shift,
+                       hist, nblocks);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, hist, 256 * nblocks);
+    hipLaunchKernelGGL(k_scatter<K>, dim3(nblocks), dim3(SORT_THREADS), 0, st, kin, vin, kout,
+                       vout, n, shift, hist, nblocks);
+    K* tk = kin; kin = kout; kout = tk;
+    uint32_t* tv = vin; vin = vout; vout = tv;
+    *result_in_tmp = !*result_in_tmp;
+  }
+  return hipGetLastError();
+}
+
+size_t radix_hist_words(int n) {
+  const int nblocks = (n + SORT_TILE - 1) / SORT_TILE;
+  return (size_t)256 * (nblocks > 0 ? nblocks : 1);
+}
+
+hipError_t radix_sort_u32(hipStream_t st, uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys,
+                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
+                          bool* result_in_tmp) {
+  return radix_sort_impl<uint32_t>(st, keys, vals, tmp_keys, tmp_vals, hist, n, bits,
+                                   result_in_tmp);
+}
+
+hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys,
+                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
+                          bool* result_in_tmp) {
+  return radix_sort_impl<uint64_t>(st, keys, vals, tmp_keys, tmp_vals, hist, n, bits,
+                                   result_in_tmp);
+}
+
+}  // namespace cs

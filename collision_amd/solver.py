@@ -1,0 +1,198 @@
+"""Python host API of the collision step, mirroring the reference's
+CollisionSolver / CollisionSolver3d (collid.h:128-244) over the C ABI.
+
+    solver = CollisionSolver3d()
+    solver.setFabricThickness(h); solver.setTimeStepSize(dt); ...
+    solver.assembleFromMesh(scene)          # assembleFromInterface
+    solver.setState(x, vel)
+    for each step:
+        solver.predict()                    # the driver's x <- x_old + dt v
+        solver.resolveCollision()           # dcollid.cpp:317-362
+
+Setters are per instance (the reference's are class statics, dcollid.cpp:56-89).
+Errors raise CollisionError carrying the C status instead of exiting the
+process as the reference's clean_up(ERROR) does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import MeshView, Params, PassInfo, StepReport
+
+
+class CollisionError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_lib.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    _lib.lib().cs_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k in ("dom_lo", "dom_hi"):
+            setattr(p, k, (C.c_double * 3)(*v))
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class CollisionSolver3d:
+    def __init__(self, params: Params | None = None, **kw):
+        self.L = _lib.lib()
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        rc = self.L.cs_create(C.byref(h), C.byref(self.params))
+        if rc:
+            raise CollisionError(rc, self.L.cs_last_error(None).decode())
+        self.h = h
+        self.nv = 0
+        self._keep = None
+        self.last_report = None
+
+    # -- lifecycle --
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow_warn=False):
+        if rc and not (allow_warn and rc == _lib.CS_WARN_ZONE_CAP):
+            raise CollisionError(rc, self.L.cs_last_error(self.h).decode())
+        return rc
+
+    def _push(self):
+        self._check(self.L.cs_set_params(self.h, C.byref(self.params)))
+
+    # -- the reference's static setters / getters (dcollid.cpp:56-89) --
+    def setRoundingTolerance(self, v): self.params.round_eps = v; self._push()
+    def getRoundingTolerance(self): return self.params.round_eps
+    def setFabricThickness(self, v): self.params.thickness = v; self._push()
+    def getFabricThickness(self): return self.params.thickness
+    def setTimeStepSize(self, v): self.params.dt = v; self._push()
+    def getTimeStepSize(self): return self.params.dt
+    def setSpringConstant(self, v): self.params.spring_k = v; self._push()
+    def getSpringConstant(self): return self.params.spring_k
+    def setFrictionConstant(self, v): self.params.friction = v; self._push()
+    def getFrictionConstant(self): return self.params.friction
+    def setPointMass(self, v): self.params.mass = v; self._push()
+    def getPointMass(self): return self.params.mass
+    def setRestitutionCoef(self, v): self.params.restitution = v; self._push()
+    def getRestitutionCoef(self): return self.params.restitution
+
+    def setDomainBoundary(self, L, U):
+        lo = np.ascontiguousarray(L, np.float64)
+        hi = np.ascontiguousarray(U, np.float64)
+        self._check(self.L.cs_set_domain_boundary(self.h, _p(lo, C.c_double), _p(hi, C.c_double)))
+        self.params.dom_lo = (C.c_double * 3)(*lo)
+        self.params.dom_hi = (C.c_double * 3)(*hi)
+
+    def getDomainBoundary(self, dir, side):
+        return (self.params.dom_lo if side == 0 else self.params.dom_hi)[dir]
+
+    # -- topology and state --
+    def assemble(self, nv, tris, tri_surf, tri_len0, bonds, bond_len0, is_fixed):
+        arrs = dict(tris=np.ascontiguousarray(tris, np.int32).reshape(-1, 3),
+                    tri_surf=np.ascontiguousarray(tri_surf, np.int32),
+                    tri_len0=np.ascontiguousarray(tri_len0, np.float64).reshape(-1, 3),
+                    bonds=np.ascontiguousarray(bonds, np.int32).reshape(-1, 2),
+                    bond_len0=np.ascontiguousarray(bond_len0, np.float64),
+                    is_fixed=np.ascontiguousarray(is_fixed, np.uint8))
+        m = MeshView(nv, arrs["tris"].shape[0], arrs["bonds"].shape[0],
+                     _p(arrs["tris"], C.c_int32), _p(arrs["tri_surf"], C.c_int32),
+                     _p(arrs["tri_len0"], C.c_double), _p(arrs["bonds"], C.c_int32),
+                     _p(arrs["bond_len0"], C.c_double), _p(arrs["is_fixed"], C.c_uint8))
+        self._check(self.L.cs_assemble(self.h, C.byref(m)))
+        self.nv = nv
+
+    def assembleFromMesh(self, scene):
+        """assembleFromInterface over a flat scene (collision_amd.scenes.Scene)."""
+        self.assemble(scene.nv, scene.tris, scene.tri_surf, scene.tri_len0, scene.bonds,
+                      scene.bond_len0, scene.fixed)
+
+    def setState(self, x, vel, has_collsn=None):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(vel, np.float64)
+        hc = None if has_collsn is None else np.ascontiguousarray(has_collsn, np.uint8)
+        self._check(self.L.cs_set_state(self.h, _p(x, C.c_double), _p(v, C.c_double),
+                                        _p(hc, C.c_uint8)))
+
+    def getState(self):
+        n = self.nv
+        out = dict(x=np.empty((n, 3)), vel=np.empty((n, 3)), avg_vel=np.empty((n, 3)),
+                   x_old=np.empty((n, 3)), has_collsn=np.empty(n, np.uint8))
+        self._check(self.L.cs_get_state(self.h, _p(out["x"], C.c_double),
+                                        _p(out["vel"], C.c_double),
+                                        _p(out["avg_vel"], C.c_double),
+                                        _p(out["x_old"], C.c_double),
+                                        _p(out["has_collsn"], C.c_uint8)))
+        return out
+
+    def recordOriginPosition(self):
+        self._check(self.L.cs_record_origin_position(self.h))
+
+    # -- step --
+    def predict(self):
+        self._check(self.L.cs_predict(self.h))
+
+    def resolveCollision(self) -> StepReport:
+        rep = StepReport()
+        self._check(self.L.cs_resolve(self.h, C.byref(rep)), allow_warn=True)
+        self.last_report = rep
+        return rep
+
+    def step(self) -> StepReport:
+        self.predict()
+        return self.resolveCollision()
+
+    def hasCollision(self) -> bool:
+        return bool(self.L.cs_has_collision(self.h))
+
+    def synchronize(self):
+        self._check(self.L.cs_synchronize(self.h))
+
+    # -- parity taps --
+    def passes(self):
+        n = self.L.cs_pass_count(self.h)
+        arr = (PassInfo * max(n, 1))()
+        self._check(self.L.cs_pass_infos(self.h, arr, n))
+        return [dict(kind=_lib.PASS_NAMES[p.kind], index=p.index, candidates=p.candidates,
+                     narrow_pairs=p.narrow_pairs, hit_pairs=p.hit_pairs,
+                     hit_subtests=p.hit_subtests, ms_broad=p.ms_broad, ms_narrow=p.ms_narrow,
+                     ms_response=p.ms_response) for p in arr[:n]]
+
+    def pairLog(self) -> np.ndarray:
+        n = self.L.cs_pair_log_count(self.h)
+        out = np.zeros((max(n, 0), 4), np.int32)
+        if n:
+            self._check(self.L.cs_pair_log(self.h, _p(out, C.c_int32), n))
+        return out
+
+    def zoneLabels(self) -> np.ndarray:
+        out = np.empty(self.nv, np.int32)
+        self._check(self.L.cs_zone_labels(self.h, _p(out, C.c_int32)))
+        return out
+
+
+def solver_for_scene(scene, **kw) -> CollisionSolver3d:
+    base = dict(dt=scene.dt, thickness=scene.thickness, round_eps=scene.round_eps,
+                spring_k=scene.spring_k, mass=scene.mass, friction=scene.friction,
+                restitution=scene.restitution, dom_lo=scene.domain_lo, dom_hi=scene.domain_hi)
+    base.update(kw)
+    s = CollisionSolver3d(**base)
+    s.assembleFromMesh(scene)
+    s.setState(scene.x, scene.vel)
+    return s

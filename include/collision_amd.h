@@ -1,0 +1,155 @@
+/*
+ * collision_amd.h -- C ABI of the MI355X (gfx950) collision step.
+ *
+ * Drop-in boundary for the reference's CollisionSolver / CollisionSolver3d
+ * (collid.h:128-244, dcollid.cpp:56-107, 317-362).  The reference mutates
+ * FronTier POINT/STATE records in place; here a handle owns device-resident
+ * copies of the same per-vertex state (Coords, x_old, vel, avgVel,
+ * has_collsn) and element topology, and the caller moves state in and out
+ * with plain host arrays.  All arrays are row-major [n][3] doubles, int32
+ * indices, uint8 flags.  No torch or HIP types appear in the signatures.
+ *
+ * Every call returns a cs_status; cs_last_error() gives text.  Handles are
+ * independent (no global state), each owns one HIP stream; calls on one
+ * handle must not overlap.
+ */
+#ifndef COLLISION_AMD_H
+#define COLLISION_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cs_status {
+    CS_OK = 0,
+    CS_ERR_ARG = 1,          /* bad argument or call order */
+    CS_ERR_NONFINITE = 2,    /* NaN/inf where the reference calls clean_up(ERROR)
+                                (dcollid.cpp:178-184, 713-718; dcollid3d.cpp:750-768,
+                                 898-902, 1093-1105, 1291-1297) */
+    CS_ERR_UNSUPPORTED = 3,  /* input outside the implemented model */
+    CS_ERR_HIP = 4,          /* HIP runtime failure */
+    CS_ERR_NOMEM = 5,
+    CS_ERR_NO_DEVICE = 6,
+    CS_ERR_INTERNAL = 7,
+    CS_WARN_ZONE_CAP = 16    /* impact-zone loop stopped by max_zone_passes; the
+                                reference loop is unbounded (dcollid.cpp:236) */
+} cs_status;
+
+typedef enum cs_pass_kind { CS_PASS_PROXIMITY = 0, CS_PASS_CCD = 1, CS_PASS_ZONE = 2 } cs_pass_kind;
+
+/* Solver parameters: the reference's static setters (dcollid.cpp:56-89)
+ * plus iteration caps. */
+typedef struct cs_params {
+    double dt;           /* setTimeStepSize       (s_dt,        default 1e-3)  */
+    double thickness;    /* setFabricThickness    (s_thickness, default 1e-4)  */
+    double round_eps;    /* setRoundingTolerance  (s_eps,       default 1e-6)  */
+    double spring_k;     /* setSpringConstant     (s_k,         default 1000)  */
+    double mass;         /* setPointMass          (s_m,         default 0.01)  */
+    double friction;     /* setFrictionConstant   (s_lambda,    default 0.02)  */
+    double restitution;  /* setRestitutionCoef    (s_cr,        default 0)     */
+    double dom_lo[3];    /* setDomainBoundary L */
+    double dom_hi[3];    /* setDomainBoundary U */
+    int32_t max_ccd_passes;     /* 5  (dcollid.cpp:433) */
+    int32_t max_zone_passes;    /* cap for the unbounded impact-zone loop */
+    int32_t max_strain_sweeps;  /* 10 (dcollid.cpp:590) */
+    int32_t log_pairs;          /* record hit pairs of every pass (parity taps) */
+    int32_t device;             /* HIP device ordinal */
+    int32_t reserved[3];
+} cs_params;
+
+/* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
+ * string curves (dcollid3d.cpp:12-52).  Element order = triangles then
+ * bonds, exactly the reference's hseList order. */
+typedef struct cs_mesh_view {
+    int32_t nv, nt, nb;
+    const int32_t *tris;      /* [nt][3] vertex ids                          */
+    const int32_t *tri_surf;  /* [nt] surface id (same-surface rigid skip)   */
+    const double *tri_len0;   /* [nt][3] rest length of edge (j, j+1)        */
+    const int32_t *bonds;     /* [nb][2] (start, end)                        */
+    const double *bond_len0;  /* [nb] rest length                            */
+    const uint8_t *is_fixed;  /* [nv] static rigid-body point (STATE.is_fixed) */
+} cs_mesh_view;
+
+typedef struct cs_pass_info {
+    int32_t kind;            /* cs_pass_kind */
+    int32_t index;
+    int64_t candidates;      /* box-overlapping element pairs */
+    int64_t narrow_pairs;    /* pairs whose sub-tests ran */
+    int64_t hit_pairs;       /* pairs whose narrow phase reported contact */
+    int64_t hit_subtests;
+    double ms_broad, ms_narrow, ms_response;
+} cs_pass_info;
+
+typedef struct cs_step_report {
+    int32_t status;
+    int32_t n_passes;
+    int32_t n_ccd_passes;
+    int32_t n_zone_passes;
+    int32_t has_collision;   /* hasCollision() */
+    int32_t n_zones;
+    int32_t strain_sweeps;
+    int32_t strain_edges;    /* edges over the limit in the last sweep */
+    int32_t degenerate_edges;
+    int32_t pad;
+    int64_t candidates, narrow_pairs, hit_pairs, hit_subtests;
+    double ms_total, ms_broad, ms_narrow, ms_response, ms_zone, ms_strain;
+} cs_step_report;
+
+typedef struct cs_solver cs_solver;
+
+/* ---- lifecycle (CollisionSolver3d(), ~CollisionSolver(), setters) ---- */
+void cs_default_params(cs_params *p);
+int cs_create(cs_solver **out, const cs_params *p);
+int cs_destroy(cs_solver *h);
+int cs_set_params(cs_solver *h, const cs_params *p);
+int cs_get_params(cs_solver *h, cs_params *p);
+/* CollisionSolver::setDomainBoundary (dcollid.cpp:109-114) */
+int cs_set_domain_boundary(cs_solver *h, const double *L, const double *U);
+
+/* ---- topology (CollisionSolver3d::assembleFromInterface) ---- */
+int cs_assemble(cs_solver *h, const cs_mesh_view *mesh);
+
+/* ---- state in / out (the FronTier POINT/STATE fields the solver reads and
+ *      writes: Coords, vel, has_collsn; x_old and avgVel readable) ---- */
+int cs_set_state(cs_solver *h, const double *x, const double *vel, const uint8_t *has_collsn);
+int cs_get_state(cs_solver *h, double *x, double *vel, double *avg_vel, double *x_old,
+                 uint8_t *has_collsn);
+/* CollisionSolver::recordOriginPosition (dcollid.cpp:91-107): x_old <- Coords,
+ * has_collsn <- false */
+int cs_record_origin_position(cs_solver *h);
+
+/* ---- step ---- */
+/* the driver's per-step predictor (test.cpp:192-258): x_old <- x,
+ * x <- x_old + dt vel */
+int cs_predict(cs_solver *h);
+/* CollisionSolver::resolveCollision (dcollid.cpp:317-362); includes the
+ * per-step union-find reset done by assembleFromInterface (makeSet).
+ * report may be NULL. */
+int cs_resolve(cs_solver *h, cs_step_report *report);
+/* cs_predict + cs_resolve */
+int cs_step(cs_solver *h, cs_step_report *report);
+/* CollisionSolver::hasCollision(): 1 if the first CCD pass of the last
+ * resolve found a collision */
+int cs_has_collision(cs_solver *h);
+int cs_synchronize(cs_solver *h);
+
+/* ---- parity taps of the last resolve ---- */
+int32_t cs_pass_count(cs_solver *h);
+int cs_pass_infos(cs_solver *h, cs_pass_info *out, int32_t cap);
+int64_t cs_pair_log_count(cs_solver *h);
+/* rows (pass, lo, hi, hit mask); order unspecified, sort before comparing */
+int cs_pair_log(cs_solver *h, int32_t *out4, int64_t cap);
+/* zone label per vertex = smallest vertex id of its impact zone */
+int cs_zone_labels(cs_solver *h, int32_t *out);
+
+/* text of the last error on this handle (or of a failed cs_create if h is NULL) */
+const char *cs_last_error(cs_solver *h);
+/* library version string */
+const char *cs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
