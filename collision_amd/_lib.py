@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("restitution", C.c_double), ("dom_lo", C.c_double * 3),
                 ("dom_hi", C.c_double * 3), ("max_ccd_passes", C.c_int32),
                 ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
-                ("log_pairs", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("log_pairs", C.c_int32), ("device", C.c_int32),
+                ("force_full_passes", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class MeshView(C.Structure):
@@ -50,7 +51,8 @@ class PassInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("candidates", C.c_int64),
                 ("narrow_pairs", C.c_int64), ("hit_pairs", C.c_int64),
                 ("hit_subtests", C.c_int64), ("ms_broad", C.c_double),
-                ("ms_narrow", C.c_double), ("ms_response", C.c_double)]
+                ("ms_narrow", C.c_double), ("ms_response", C.c_double),
+                ("ms_traverse_kernel", C.c_double), ("ms_narrow_kernel", C.c_double)]
 
 
 class StepReport(C.Structure):

@@ -24,10 +24,15 @@ using namespace cs;
 
 namespace {
 
-constexpr int CNT_ERR = 15;
-constexpr int CNT_STRAIN_SWEEPS = 11;
-constexpr int COUNTER_INTS = 32;  // ints 0..15, u64 candidate count at ints 16..17
+constexpr int COUNTER_INTS = CNT_INTS;
 std::string g_create_error;
+
+// sizes of the previous pass's candidate stream, hits and hit pairs
+struct PrevPass {
+  bool valid = false;  // previous pass was a CCD / zone pass of this step
+  int npairs = 0, nhits = 0, nhp = 0;
+  unsigned long long cand = 0;
+};
 
 }  // namespace
 
@@ -44,6 +49,7 @@ struct cs_solver {
   int64_t plog_rows = 0;
   std::vector<hipEvent_t> events;
   size_t ev_used = 0;
+  PrevPass prev;
 };
 
 namespace {
@@ -80,21 +86,41 @@ void free_dev(DevState& S) {
   dfree(S.ev); dfree(S.eflags); dfree(S.len0); dfree(S.ebox);
   dfree(S.bounds); dfree(S.mcode); dfree(S.mid); dfree(S.mcode_tmp); dfree(S.mid_tmp);
   dfree(S.sort_hist); dfree(S.nodes); dfree(S.leaves); dfree(S.parent); dfree(S.lparent);
-  dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
+  dfree(S.rmin); dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
+  dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
+  dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
+  dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist); dfree(S.tile_dirty);
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
   dfree(S.zkey); dfree(S.zkey_tmp); dfree(S.zval); dfree(S.zval_tmp); dfree(S.zsize);
+  dfree(S.zhead); dfree(S.zend);
   dfree(S.lvl_ptr); dfree(S.lvl_elems); dfree(S.plog); dfree(S.counters);
-  S.err = nullptr;
-  S.cand_count = nullptr;
+  dfree(S.elem_level); dfree(S.next_level); dfree(S.elem_active); dfree(S.lvl_active);
+  dfree(S.vdirty);
   S = DevState();
 }
 
-hipError_t alloc_hits(DevState& S, int cap) {
-  dfree(S.hits); dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
+// grow a device buffer, keeping its first `keep` entries
+template <typename T>
+hipError_t grow_keep(T** p, int keep, size_t new_count, hipStream_t st) {
+  T* np = nullptr;
+  CS_HIP_RET(dalloc(&np, new_count));
+  if (keep > 0 && *p)
+    CS_HIP_RET(hipMemcpyAsync(np, *p, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice, st));
+  CS_HIP_RET(hipStreamSynchronize(st));
+  dfree(*p);
+  *p = np;
+  return hipSuccess;
+}
+
+// hit records: current buffer keeps its first `keep` records
+hipError_t alloc_hits(DevState& S, int cap, int keep, hipStream_t st) {
+  dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
+  dfree(S.hits_prev);
   S.hit_cap = cap;
-  CS_HIP_RET(dalloc(&S.hits, cap));
+  CS_HIP_RET(grow_keep(&S.hits, keep, cap, st));
+  CS_HIP_RET(dalloc(&S.hits_prev, cap));
   CS_HIP_RET(dalloc(&S.hkey, cap));
   CS_HIP_RET(dalloc(&S.hkey_tmp, cap));
   CS_HIP_RET(dalloc(&S.hidx, cap));
@@ -111,7 +137,10 @@ hipError_t alloc_sort_hist(DevState& S) {
   const size_t words = radix_hist_words(nmax);
   if (words <= S.sort_hist_words) return hipSuccess;
   dfree(S.sort_hist);
+  dfree(S.scan_part);
   S.sort_hist_words = words;
+  S.scan_part_words = scan_partial_words((int)words);
+  CS_HIP_RET(dalloc(&S.scan_part, S.scan_part_words));
   return dalloc(&S.sort_hist, words);
 }
 
@@ -165,35 +194,118 @@ int device_error(cs_solver* h) {
 }
 
 struct PassTimes {
-  hipEvent_t e0, e1, e2, e3;
+  hipEvent_t e0, e1, e2, e3, t0, t1, n0, n1;
 };
 
-// One broad + narrow + response pass.  Returns status; *any_hit set.
+unsigned long long c64(const cs_solver* h, int slot) {
+  return *(const unsigned long long*)(h->h_counters + slot);
+}
+
+int set_counter(cs_solver* h, int slot, int value) {
+  HIPCHK(h, hipMemcpyAsync(h->S.counters + slot, &value, sizeof(int), hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));  // `value` is a host stack variable
+  return CS_OK;
+}
+
+template <typename T>
+void swap_ptr(T*& a, T*& b) {
+  T* t = a;
+  a = b;
+  b = t;
+}
+
+// One broad + narrow + response pass.  *any_hit = some pair reported contact.
+//
+// Full pass (proximity pass, first CCD pass of a step): element boxes, LBVH
+// build (proximity) or refit, traversal of every leaf, narrow phase of the
+// whole candidate stream.
+// Incremental pass (later CCD and impact-zone passes): dirty elements are
+// re-boxed and re-queried, clean pairs / hits / hit pairs are carried over,
+// and only the queried pairs go through the narrow phase.
 int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   DevState& S = h->S;
   const cs_params& P = h->P;
   const int ccd = kind != CS_PASS_PROXIMITY;
+  const bool incremental = ccd && h->prev.valid && !P.force_full_passes;
   PassTimes T{};
+  // the previous pass's results become "prev"; this pass fills the others
+  swap_ptr(S.pairs, S.pairs_prev);
+  swap_ptr(S.hits, S.hits_prev);
+  swap_ptr(S.hp, S.hp_prev);
+  const PrevPass prev = h->prev;
+  h->prev.valid = false;
   mark(h, &T.e0);
-  launch_element_boxes(h->st, S, ccd, P.dt);
-  if (kind == CS_PASS_PROXIMITY) HIPCHK(h, build_bvh(h->st, S));
-  HIPCHK(h, refit_bvh(h->st, S));
-  // traversal, re-run with a larger stream if it overflowed
-  int npairs = 0;
-  for (;;) {
-    HIPCHK(h, hipMemsetAsync(S.counters + CNT_PAIRS, 0, sizeof(int), h->st));
-    HIPCHK(h, hipMemsetAsync(S.cand_count, 0, sizeof(unsigned long long), h->st));
-    launch_traverse(h->st, S, S.pair_cap);
+  HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * 4, h->st));            // pairs, hits, hp, sub
+  HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
+  HIPCHK(h, hipMemsetAsync(S.counters + CNT_DIRTY, 0, sizeof(int), h->st));
+  HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 6 * sizeof(int), h->st));
+  int npairs = 0, carried_hits = 0, carried_hp = 0, carried_sub = 0;
+  unsigned long long cand = 0;
+  int nq = 0;
+  if (!incremental) {
+    HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
+    launch_element_boxes(h->st, S, ccd, P.dt);
+    if (kind == CS_PASS_PROXIMITY) HIPCHK(h, build_bvh(h->st, S));
+    HIPCHK(h, refit_bvh(h->st, S));
+    for (;;) {  // traversal; re-run with a larger stream if it overflowed
+      mark(h, &T.t0);
+      launch_traverse(h->st, S, S.pair_cap);
+      mark(h, &T.t1);
+      HIPCHK(h, hipGetLastError());
+      int rc = read_counters(h);
+      if (rc) return rc;
+      npairs = h->h_counters[CNT_PAIRS];
+      if (npairs <= S.pair_cap) break;
+      S.pair_cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
+      dfree(S.pairs); dfree(S.pairs_prev);
+      HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
+      HIPCHK(h, dalloc(&S.pairs_prev, S.pair_cap));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_PAIRS, 0, sizeof(int), h->st));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 2 * sizeof(int), h->st));
+    }
+    cand = c64(h, CNT_CAND64);
+  } else {
+    launch_mark_dirty(h->st, S);
+    HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
+    mark(h, &T.t0);
+    launch_query_dirty(h->st, S, 0);                 // previous boxes: count only
+    launch_dirty_element_boxes(h->st, S, P.dt);
+    HIPCHK(h, refit_bvh(h->st, S, true));
+    launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
+    mark(h, &T.t1);
+    launch_carry(h->st, S, prev.npairs, prev.nhits, prev.nhp);
     HIPCHK(h, hipGetLastError());
     int rc = read_counters(h);
     if (rc) return rc;
-    npairs = h->h_counters[CNT_PAIRS];
-    if (npairs <= S.pair_cap) break;
-    dfree(S.pairs);
-    S.pair_cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
-    HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
+    nq = h->h_counters[CNT_QPAIRS];
+    if (nq > S.qpair_cap) {  // grow and query again
+      S.qpair_cap = nq + nq / 2 + 1024;
+      dfree(S.qpairs);
+      HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_NEW64, 0, 2 * sizeof(int), h->st));
+      launch_query_dirty(h->st, S, 1);
+      rc = read_counters(h);
+      if (rc) return rc;
+      nq = h->h_counters[CNT_QPAIRS];
+    }
+    const int carried_pairs = h->h_counters[CNT_PAIRS];
+    carried_hits = h->h_counters[CNT_HITS];
+    carried_hp = h->h_counters[CNT_HIT_PAIRS];
+    carried_sub = h->h_counters[CNT_HIT_SUB];
+    npairs = carried_pairs + nq;
+    if (npairs > S.pair_cap) {
+      const int cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
+      HIPCHK(h, grow_keep(&S.pairs, carried_pairs, cap, h->st));
+      dfree(S.pairs_prev);
+      HIPCHK(h, dalloc(&S.pairs_prev, cap));
+      S.pair_cap = cap;
+    }
+    if (nq > 0)
+      HIPCHK(h, hipMemcpyAsync(S.pairs + carried_pairs, S.qpairs, sizeof(int2) * (size_t)nq,
+                               hipMemcpyDeviceToDevice, h->st));
+    cand = prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64);
   }
-  const unsigned long long cand = *(unsigned long long*)(h->h_counters + 16);
   mark(h, &T.e1);
   NarrowParams NP;
   NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
@@ -202,31 +314,44 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   NP.zone_mode = (kind == CS_PASS_ZONE);
   NP.log_pairs = P.log_pairs;
   NP.pass_id = pass_id;
-  int nhits = 0;
+  const int2* narrow_in = incremental ? S.qpairs : S.pairs;
+  const int narrow_n = incremental ? nq : npairs;
+  int nhits = 0, nhp = 0;
   for (;;) {
-    HIPCHK(h, hipMemsetAsync(S.counters + CNT_HITS, 0, sizeof(int) * 3, h->st));
-    launch_narrow(h->st, S, NP, npairs);
+    mark(h, &T.n0);
+    launch_narrow(h->st, S, NP, narrow_in, narrow_n);
+    mark(h, &T.n1);
     HIPCHK(h, hipGetLastError());
     int rc = read_counters(h);
     if (rc) return rc;
     nhits = h->h_counters[CNT_HITS];
-    if (nhits <= S.hit_cap) break;
-    // grow and re-run: the narrow phase only writes hit records, counters
-    // and (idempotent) zone unions
-    if (P.log_pairs) {
-      // rows of this pass are written again by the re-run; drop them
-      int keep = (int)h->plog_rows;
-      HIPCHK(h, hipMemcpyAsync(S.counters + CNT_LOG, &keep, sizeof(int), hipMemcpyHostToDevice, h->st));
+    nhp = h->h_counters[CNT_HIT_PAIRS];
+    if (nhits <= S.hit_cap && nhp <= S.hp_cap) break;
+    // grow (keeping carried entries) and run the narrow phase again; it only
+    // appends records and performs idempotent zone unions
+    if (nhits > S.hit_cap) {
+      HIPCHK(h, alloc_hits(S, nhits + nhits / 2 + 1024, carried_hits, h->st));
+      HIPCHK(h, alloc_sort_hist(S));
     }
-    HIPCHK(h, alloc_hits(S, nhits + nhits / 2 + 1024));
-    HIPCHK(h, alloc_sort_hist(S));
+    if (nhp > S.hp_cap) {
+      const int cap = nhp + nhp / 2 + 1024;
+      HIPCHK(h, grow_keep(&S.hp, carried_hp, cap, h->st));
+      dfree(S.hp_prev);
+      HIPCHK(h, dalloc(&S.hp_prev, cap));
+      S.hp_cap = cap;
+    }
+    if ((rc = set_counter(h, CNT_HITS, carried_hits))) return rc;
+    if ((rc = set_counter(h, CNT_HIT_PAIRS, carried_hp))) return rc;
+    if ((rc = set_counter(h, CNT_HIT_SUB, carried_sub))) return rc;
   }
+  if (incremental && kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, carried_hp, 1);
   mark(h, &T.e2);
   HIPCHK(h, apply_hits(h->st, S, nhits));
   if (kind == CS_PASS_ZONE) {
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), h->st));
     HIPCHK(h, update_zones(h->st, S, P.mass, P.dt));
   }
+  if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark(h, &T.e3);
   int rc = read_counters(h);
   if (rc) return rc;
@@ -236,13 +361,22 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   pi.index = index;
   pi.candidates = (int64_t)cand;
   pi.narrow_pairs = npairs;
-  pi.hit_pairs = h->h_counters[CNT_HIT_PAIRS];
+  pi.hit_pairs = nhp;
   pi.hit_subtests = h->h_counters[CNT_HIT_SUB];
   pi.ms_broad = elapsed(T.e0, T.e1);
   pi.ms_narrow = elapsed(T.e1, T.e2);
   pi.ms_response = elapsed(T.e2, T.e3);
+  pi.ms_traverse_kernel = elapsed(T.t0, T.t1);
+  pi.ms_narrow_kernel = elapsed(T.n0, T.n1);
   h->passes.push_back(pi);
-  *any_hit = pi.hit_pairs > 0;
+  *any_hit = nhp > 0;
+  if (ccd) {
+    h->prev.valid = true;
+    h->prev.npairs = npairs;
+    h->prev.nhits = nhits;
+    h->prev.nhp = nhp;
+    h->prev.cand = cand;
+  }
   return device_error(h);
 }
 
@@ -392,6 +526,20 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     nlev = std::max(nlev, l + 1);
     ++nsched;
   }
+  // level of each vertex's next visit (elements in order, rigid ones skipped)
+  std::vector<int4> nextl((size_t)std::max(ne, 1), make_int4(-1, -1, -1, -1));
+  {
+    std::vector<int> nxt((size_t)nv, -1);
+    for (int e = ne - 1; e >= 0; --e) {
+      if (level[e] < 0) continue;
+      const int np = (ef[e] & EF_TRI) ? 3 : 2;
+      const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
+      int nl[3] = {-1, -1, -1};
+      for (int i = 0; i < np; ++i) nl[i] = nxt[vv[i]];
+      nextl[e] = make_int4(nl[0], nl[1], nl[2], -1);
+      for (int i = 0; i < np; ++i) nxt[vv[i]] = level[e];
+    }
+  }
   std::vector<int> lptr((size_t)nlev + 1, 0), lel((size_t)std::max(nsched, 1));
   for (int e = 0; e < ne; ++e)
     if (level[e] >= 0) lptr[level[e] + 1]++;
@@ -417,20 +565,35 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.mcode_tmp, ne)); HIPCHK(h, dalloc(&S.mid_tmp, ne));
   HIPCHK(h, dalloc(&S.nodes, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.leaves, ne));
   HIPCHK(h, dalloc(&S.parent, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.lparent, ne));
-  HIPCHK(h, dalloc(&S.rmax, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.arrive, std::max(ne - 1, 1)));
-  S.pair_cap = (int)std::min<int64_t>(8ll * ne + 4096, INT32_MAX / 2);
+  HIPCHK(h, dalloc(&S.rmin, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.rmax, std::max(ne - 1, 1)));
+  HIPCHK(h, dalloc(&S.arrive, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.top_start, std::max(ne - 1, 1)));
+  S.pair_cap = (int)std::min<int64_t>(3ll * ne + 4096, INT32_MAX / 2);
   HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
-  HIPCHK(h, alloc_hits(S, std::max(1 << 16, ne / 4)));
+  HIPCHK(h, dalloc(&S.pairs_prev, S.pair_cap));
+  S.qpair_cap = std::max(1 << 16, ne / 8);
+  HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
+  HIPCHK(h, alloc_hits(S, std::max(1 << 16, ne / 4), 0, h->st));
+  S.hp_cap = std::max(1 << 16, ne / 4);
+  HIPCHK(h, dalloc(&S.hp, S.hp_cap));
+  HIPCHK(h, dalloc(&S.hp_prev, S.hp_cap));
+  HIPCHK(h, dalloc(&S.leaf_of, ne));
+  HIPCHK(h, dalloc(&S.dirty_v, nv));
+  HIPCHK(h, dalloc(&S.dirty_e, ne));
+  HIPCHK(h, dalloc(&S.dlist, ne));
+  S.n_tiles = refit_tile_count(ne);
+  HIPCHK(h, dalloc(&S.tile_dirty, S.n_tiles));
   HIPCHK(h, dalloc(&S.zkey, nv)); HIPCHK(h, dalloc(&S.zkey_tmp, nv));
   HIPCHK(h, dalloc(&S.zval, nv)); HIPCHK(h, dalloc(&S.zval_tmp, nv)); HIPCHK(h, dalloc(&S.zsize, nv));
+  HIPCHK(h, dalloc(&S.zhead, nv)); HIPCHK(h, dalloc(&S.zend, nv));
   S.n_levels = nlev;
   S.n_sched = nsched;
   HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1)); HIPCHK(h, dalloc(&S.lvl_elems, std::max(nsched, 1)));
+  HIPCHK(h, dalloc(&S.elem_level, ne)); HIPCHK(h, dalloc(&S.next_level, ne));
+  HIPCHK(h, dalloc(&S.elem_active, ne)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
+  HIPCHK(h, dalloc(&S.vdirty, nv));
   S.plog_cap = 1 << 20;
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
   HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
-  S.err = S.counters + CNT_ERR;
-  S.cand_count = (unsigned long long*)(S.counters + 16);
   HIPCHK(h, alloc_sort_hist(S));
   HIPCHK(h, hipMemcpy(S.ev, ev.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.eflags, ef.data(), ne, hipMemcpyHostToDevice));
@@ -438,8 +601,11 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.fixed, fixed.data(), nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.lvl_ptr, lptr.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
   if (nsched) HIPCHK(h, hipMemcpy(S.lvl_elems, lel.data(), sizeof(int) * nsched, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.elem_level, level.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.next_level, nextl.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
   HIPCHK(h, hipMemset(S.has, 0, nv));
+  HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
   h->assembled = true;
   h->state_set = false;
   return CS_OK;
@@ -510,6 +676,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   h->plog_rows = 0;
   h->ev_used = 0;
   h->has_collision = 0;
+  h->prev = PrevPass();
   hipEvent_t t0, t_strain0, t_end;
   mark(h, &t0);
   HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));

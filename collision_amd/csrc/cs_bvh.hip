@@ -10,14 +10,16 @@
 //   * k_bounds / k_morton : 30-bit Morton codes of FP32 box centres.
 //   * (radix sort of codes, cs_sort.hip)
 //   * k_karras        : radix tree over the sorted codes (index tie-break).
-//   * k_refit         : bottom-up FP32 boxes rounded outward, so every FP64
-//     overlap is also an FP32 overlap; arrival counters order the updates.
+//   * k_refit_tiles / k_refit_top : bottom-up FP32 boxes rounded outward,
+//     so every FP64 overlap is also an FP32 overlap.
 //   * k_traverse      : one lane per leaf in Morton order walks the tree and
 //     only visits leaves after itself in that order (each pair once); a
 //     leaf overlap is confirmed with the exact FP64 boxes, then the
 //     reference's early rejects (shared vertex, same rigid surface;
 //     dcollid.cpp:762, 805; dcollid3d.cpp:209-214, 257-264, 279-284) run
 //     before the pair is appended to the compacted candidate stream.
+#include <algorithm>
+
 #include "cs_internal.h"
 
 namespace cs {
@@ -141,7 +143,7 @@ __device__ inline int lcp_delta(const uint32_t* __restrict__ codes, int n, int i
 // Karras (2012) radix tree: internal nodes 0..n-2; child < 0 is leaf ~j.
 __global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __restrict__ nodes,
                          int32_t* __restrict__ parent, int32_t* __restrict__ lparent,
-                         int32_t* __restrict__ rmax) {
+                         int32_t* __restrict__ rmin, int32_t* __restrict__ rmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (lcp_delta(codes, n, i, i + 1) - lcp_delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -165,47 +167,115 @@ __global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __
   const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
   nodes[i].left = left;
   nodes[i].right = right;
+  rmin[i] = lo;
   rmax[i] = hi;
   if (left < 0) lparent[~left] = i; else parent[left] = i;
   if (right < 0) lparent[~right] = i; else parent[right] = i;
   if (i == 0) parent[0] = -1;
 }
 
-// Leaf boxes (sorted order) rounded outward to FP32, then internal nodes
-// bottom-up: the second lane to arrive at a node merges both children.
-__global__ void k_refit(const dbox* __restrict__ ebox, const uint32_t* __restrict__ sorted_ids,
-                        int n, bvh_node* __restrict__ leaves, bvh_node* __restrict__ nodes,
-                        const int32_t* __restrict__ parent, const int32_t* __restrict__ lparent,
-                        uint32_t* __restrict__ arrive) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int e = (int)sorted_ids[j];
-  const dbox b = ebox[e];
-  bvh_node lf;
+// Refit.  Every internal node covers a contiguous range [rmin, rmax] of
+// Morton-sorted leaves.  Nodes whose range lies inside one tile of
+// REFIT_TILE leaves are refit by that tile's work-group, bottom-up: the
+// second lane to arrive at a node (work-group-scope atomic counter) merges
+// both children.  The remaining "top" nodes are refit the same way by a
+// single work-group, starting from the top nodes whose children are all
+// tile-local.  No agent-scope fence is needed: every counter is touched by
+// one work-group only, and data crosses work-groups only at kernel
+// boundaries.  Leaf boxes are the FP64 boxes rounded outward to FP32.
+constexpr int REFIT_THREADS = 1024;
+constexpr int REFIT_TILE = 8192;
+
+int refit_tile_count(int n) { return (n + REFIT_TILE - 1) / REFIT_TILE; }
+
+__device__ inline bool is_top(const int32_t* rmin, const int32_t* rmax, int node) {
+  return (rmin[node] / REFIT_TILE) != (rmax[node] / REFIT_TILE);
+}
+
+__device__ inline void merge_children(bvh_node* nodes, const bvh_node* leaves, int node) {
+  const int l = nodes[node].left, r = nodes[node].right;
+  const bvh_node A = (l < 0) ? leaves[~l] : nodes[l];
+  const bvh_node B = (r < 0) ? leaves[~r] : nodes[r];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    lf.lo[d] = f32_down(b.lo[d]);
-    lf.hi[d] = f32_up(b.hi[d]);
+    nodes[node].lo[d] = fminf(A.lo[d], B.lo[d]);
+    nodes[node].hi[d] = fmaxf(A.hi[d], B.hi[d]);
   }
-  lf.left = e;
-  lf.right = j;
-  leaves[j] = lf;
-  if (n == 1) return;
-  __threadfence();
-  int node = lparent[j];
-  while (node >= 0) {
-    if (atomicAdd(&arrive[node], 1u) == 0u) return;  // first arrival stops
-    __threadfence();
-    const int l = nodes[node].left, r = nodes[node].right;
-    const bvh_node A = (l < 0) ? leaves[~l] : nodes[l];
-    const bvh_node B = (r < 0) ? leaves[~r] : nodes[r];
+}
+
+__device__ inline uint32_t wg_arrive(uint32_t* p) {
+  return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__global__ void __launch_bounds__(REFIT_THREADS)
+    k_refit_tiles(const dbox* __restrict__ ebox, const uint32_t* __restrict__ sorted_ids, int n,
+                  bvh_node* leaves, bvh_node* nodes, const int32_t* __restrict__ parent,
+                  const int32_t* __restrict__ lparent, const int32_t* __restrict__ rmin,
+                  const int32_t* __restrict__ rmax, uint32_t* arrive,
+                  const uint8_t* __restrict__ tile_dirty) {
+  if (tile_dirty && !tile_dirty[blockIdx.x]) return;  // boxes of this tile are unchanged
+  const int first = blockIdx.x * REFIT_TILE;
+  const int last = min(n, first + REFIT_TILE);
+  for (int j = first + threadIdx.x; j < last; j += REFIT_THREADS) {
+    const int e = (int)sorted_ids[j];
+    const dbox b = ebox[e];
+    bvh_node lf;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      nodes[node].lo[d] = fminf(A.lo[d], B.lo[d]);
-      nodes[node].hi[d] = fmaxf(A.hi[d], B.hi[d]);
+      lf.lo[d] = f32_down(b.lo[d]);
+      lf.hi[d] = f32_up(b.hi[d]);
     }
-    __threadfence();
+    lf.left = e;
+    lf.right = j;
+    leaves[j] = lf;
+    if (n == 1) continue;
+    int node = lparent[j];
+    while (node >= 0 && !is_top(rmin, rmax, node)) {
+      if (wg_arrive(&arrive[node]) == 0u) break;  // first arrival: sibling not done
+      merge_children(nodes, leaves, node);
+      node = parent[node];
+    }
+  }
+}
+
+// top nodes with no top child: start points of the single-work-group refit
+__global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __restrict__ rmin,
+                           const int32_t* __restrict__ rmax, int n, int32_t* __restrict__ out,
+                           int* __restrict__ count) {
+  __shared__ int s_cnt, s_base;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool start = false;
+  if (i < n - 1 && is_top(rmin, rmax, i)) {
+    const int l = nodes[i].left, r = nodes[i].right;
+    start = !(l >= 0 && is_top(rmin, rmax, l)) && !(r >= 0 && is_top(rmin, rmax, r));
+  }
+  int slot = -1;
+  if (start) slot = atomicAdd(&s_cnt, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(count, s_cnt) : 0;
+  __syncthreads();
+  if (start) out[s_base + slot] = i;
+}
+
+__global__ void __launch_bounds__(1024)
+    k_refit_top(bvh_node* nodes, const bvh_node* leaves, const int32_t* __restrict__ parent,
+                const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
+                const int32_t* __restrict__ starts, int nstarts, uint32_t* arrive) {
+  for (int k = threadIdx.x; k < nstarts; k += 1024) {
+    int node = starts[k];
+    merge_children(nodes, leaves, node);
     node = parent[node];
+    while (node >= 0) {
+      // arrivals needed = number of top children (the others are done)
+      const int l = nodes[node].left, r = nodes[node].right;
+      const uint32_t need = (uint32_t)((l >= 0 && is_top(rmin, rmax, l)) ? 1 : 0) +
+                            (uint32_t)((r >= 0 && is_top(rmin, rmax, r)) ? 1 : 0);
+      if (wg_arrive(&arrive[node]) + 1u < need) break;
+      merge_children(nodes, leaves, node);
+      node = parent[node];
+    }
   }
 }
 
@@ -250,19 +320,27 @@ __device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
 }
 
 constexpr int TRAV_STACK = 64;
+constexpr int TRAV_THREADS = 256;
+constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flush
 
-__global__ void __launch_bounds__(256) k_traverse(const bvh_node* __restrict__ nodes,
-                                                  const bvh_node* __restrict__ leaves,
-                                                  const int32_t* __restrict__ rmax,
-                                                  const dbox* __restrict__ ebox,
-                                                  const int4* __restrict__ ev,
-                                                  const uint8_t* __restrict__ eflags, int n,
-                                                  int2* __restrict__ pairs, int cap,
-                                                  int* __restrict__ pair_count,
-                                                  unsigned long long* __restrict__ cand_count,
-                                                  int* __restrict__ err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
+                                                           const bvh_node* __restrict__ leaves,
+                                                           const int32_t* __restrict__ rmax,
+                                                           const dbox* __restrict__ ebox,
+                                                           const int4* __restrict__ ev,
+                                                           const uint8_t* __restrict__ eflags,
+                                                           int n, int2* __restrict__ pairs, int cap,
+                                                           int* __restrict__ pair_count,
+                                                           unsigned long long* __restrict__ cand_count,
+                                                           int* __restrict__ err) {
+  __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
+  __shared__ int s_off[TRAV_THREADS];
+  __shared__ unsigned long long s_cand[TRAV_THREADS / 64];
+  __shared__ int s_base;
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * TRAV_THREADS + t;
   unsigned long long cand = 0;
+  int nbuf = 0;
   if (i < n && n > 1) {
     const bvh_node me = leaves[i];
     const int e = me.left;
@@ -289,11 +367,16 @@ __global__ void __launch_bounds__(256) k_traverse(const bvh_node* __restrict__ n
           if (!f64_overlap(ob, mb)) continue;
           ++cand;
           const int4 ot = ev[e2];
-          const int hi = max(e, e2), lo = min(e, e2);
           const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
           if (rej) continue;
-          const int slot = atomicAdd(pair_count, 1);
-          if (slot < cap) pairs[slot] = make_int2(lo, hi);
+          const int2 pr = make_int2(min(e, e2), max(e, e2));
+          if (nbuf < TRAV_BUF) {
+            buf[nbuf * TRAV_THREADS + t] = pr;
+            ++nbuf;
+          } else {
+            const int slot = atomicAdd(pair_count, 1);
+            if (slot < cap) pairs[slot] = pr;
+          }
         } else {
           if (rmax[cc] <= i) continue;
           if (!f32_overlap(nodes[cc], me.lo, me.hi)) continue;
@@ -306,9 +389,300 @@ __global__ void __launch_bounds__(256) k_traverse(const bvh_node* __restrict__ n
       }
     }
   }
-  // one 64-bit atomic per wavefront for the candidate count
+  // block-wide exclusive scan of the buffered counts, one atomic per block
+  s_off[t] = nbuf;
   for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off);
-  if ((threadIdx.x & 63) == 0 && cand) atomicAdd(cand_count, cand);
+  if ((t & 63) == 0) s_cand[t >> 6] = cand;
+  __syncthreads();
+  for (int off = 1; off < TRAV_THREADS; off <<= 1) {
+    const int add = (t >= off) ? s_off[t - off] : 0;
+    __syncthreads();
+    s_off[t] += add;
+    __syncthreads();
+  }
+  if (t == TRAV_THREADS - 1) {
+    const int total = s_off[t];
+    s_base = total ? atomicAdd(pair_count, total) : 0;
+    unsigned long long c = 0;
+    for (int w = 0; w < TRAV_THREADS / 64; ++w) c += s_cand[w];
+    if (c) atomicAdd(cand_count, c);
+  }
+  __syncthreads();
+  const int base = s_base + s_off[t] - nbuf;
+  for (int k = 0; k < nbuf; ++k)
+    if (base + k < cap) pairs[base + k] = buf[k * TRAV_THREADS + t];
+}
+
+// ---------------------------------------------------------------------------
+// Incremental CCD / impact-zone passes.  Between two passes of one step only
+// the average velocities of vertices touched by the last apply or zone update
+// change ("dirty" vertices).  A pair whose two elements have no dirty vertex
+// keeps its swept boxes, its candidate status and its narrow-phase result
+// bit for bit, so the next pass carries those over and only
+//   * re-queries dirty elements against the refit tree,
+//   * re-runs the narrow phase on pairs that touch a dirty element.
+// Candidate counts stay exact: count = previous - (pairs touching dirty
+// elements under the previous boxes) + (the same under the new boxes).
+
+__global__ void k_leaf_of(const uint32_t* __restrict__ sorted_ids, int n, int32_t* __restrict__ leaf_of) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) leaf_of[sorted_ids[j]] = j;
+}
+
+// dirty_e[e] = element has a dirty vertex; list of dirty elements
+__global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
+                                                    const uint8_t* __restrict__ dirty_v, int ne,
+                                                    const int32_t* __restrict__ leaf_of,
+                                                    uint8_t* __restrict__ dirty_e,
+                                                    uint8_t* __restrict__ tile_dirty,
+                                                    int32_t* __restrict__ dlist,
+                                                    int* __restrict__ counters) {
+  __shared__ int s_cnt, s_base;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  bool d = false;
+  if (e < ne) {
+    const int4 t = ev[e];
+    d = dirty_v[t.x] | dirty_v[t.y] | ((t.z >= 0) ? dirty_v[t.z] : 0);
+    dirty_e[e] = d ? 1 : 0;
+    if (d) tile_dirty[leaf_of[e] / REFIT_TILE] = 1;
+  }
+  const int slot = block_append(d, &counters[CNT_DIRTY], &s_cnt, &s_base);
+  if (d) dlist[slot] = e;
+}
+
+__global__ void k_dirty_boxes(const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
+                              const dvec4* __restrict__ av, double dt,
+                              const int32_t* __restrict__ dlist, const int* __restrict__ counters,
+                              dbox* __restrict__ ebox) {
+  const int nd = counters[CNT_DIRTY];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
+  const int e = dlist[k];
+  const int4 t = ev[e];
+  const int np = (t.z >= 0) ? 3 : 2;
+  const int vid[3] = {t.x, t.y, t.z};
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+  double hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int i = 0; i < np; ++i) {
+    const dvec4 xo = x_old[vid[i]];
+    const dvec4 a = av[vid[i]];
+    const double o[3] = {xo.x, xo.y, xo.z};
+    const double nn[3] = {xo.x + a.x * dt, xo.y + a.y * dt, xo.z + a.z * dt};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      hi[d] = smax(hi[d], o[d]);
+      hi[d] = smax(hi[d], nn[d]);
+      lo[d] = smin(lo[d], o[d]);
+      lo[d] = smin(lo[d], nn[d]);
+    }
+  }
+  dbox b;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = lo[d] - CS_BOX_EPS;
+    b.hi[d] = hi[d] + CS_BOX_EPS;
+  }
+  ebox[e] = b;
+  }
+}
+
+// Query of every dirty element against the whole tree.  A pair of two dirty
+// elements is taken from the one earlier in Morton order.  emit = 0: count
+// only (previous boxes, into CNT_OLD64); emit = 1: count into CNT_NEW64 and
+// append non-rejected pairs to qpairs.
+__global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
+    const bvh_node* __restrict__ nodes, const bvh_node* __restrict__ leaves,
+    const dbox* __restrict__ ebox, const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
+    const uint8_t* __restrict__ dirty_e, const int32_t* __restrict__ dlist,
+    const int32_t* __restrict__ leaf_of, int n, int emit, int2* __restrict__ qpairs, int qcap,
+    int* __restrict__ counters) {
+  __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
+  __shared__ int s_base;
+  __shared__ unsigned long long s_w[TRAV_THREADS / 64];
+  __shared__ int s_off[TRAV_THREADS];
+  const int t = threadIdx.x;
+  const int ndirty = counters[CNT_DIRTY];
+  const int rounds = (ndirty + (int)(gridDim.x * TRAV_THREADS) - 1) / (int)(gridDim.x * TRAV_THREADS);
+  for (int round = 0; round < rounds; ++round) {
+  const int k = (round * (int)gridDim.x + (int)blockIdx.x) * TRAV_THREADS + t;
+  unsigned long long cand = 0;
+  int nbuf = 0;
+  if (k < ndirty && n > 1) {
+    const int e = dlist[k];
+    const int si = leaf_of[e];
+    const bvh_node me = leaves[si];
+    const dbox mb = ebox[e];
+    const int4 mt = ev[e];
+    const uint8_t mf = eflags[e];
+    int stack[TRAV_STACK];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+      const int nd = stack[--sp];
+      const bvh_node node = nodes[nd];
+      const int ch[2] = {node.left, node.right};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int cc = ch[c];
+        if (cc < 0) {
+          const int j = ~cc;
+          if (j == si) continue;
+          const bvh_node lf = leaves[j];
+          if (!f32_overlap(lf, me.lo, me.hi)) continue;
+          const int e2 = lf.left;
+          if (dirty_e[e2] && j < si) continue;  // counted from e2
+          const dbox ob = ebox[e2];
+          if (!f64_overlap(ob, mb)) continue;
+          ++cand;
+          if (!emit) continue;
+          const int4 ot = ev[e2];
+          const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
+          if (rej) continue;
+          const int2 pr = make_int2(min(e, e2), max(e, e2));
+          if (nbuf < TRAV_BUF) {
+            buf[nbuf * TRAV_THREADS + t] = pr;
+            ++nbuf;
+          } else {
+            const int slot = atomicAdd(&counters[CNT_QPAIRS], 1);
+            if (slot < qcap) qpairs[slot] = pr;
+          }
+        } else {
+          if (!f32_overlap(nodes[cc], me.lo, me.hi)) continue;
+          if (sp < TRAV_STACK) stack[sp++] = cc;
+          else atomicOr(&counters[CNT_ERR], CS_DEVERR_STACK);
+        }
+      }
+    }
+  }
+  const unsigned long long c = block_sum_u64(cand, s_w);
+  if (t == 0 && c)
+    atomicAdd((unsigned long long*)(counters + (emit ? CNT_NEW64 : CNT_OLD64)), c);
+  if (!emit) continue;
+  // append the buffered pairs: exclusive scan of nbuf over the block
+  s_off[t] = nbuf;
+  __syncthreads();
+  for (int off = 1; off < TRAV_THREADS; off <<= 1) {
+    const int add = (t >= off) ? s_off[t - off] : 0;
+    __syncthreads();
+    s_off[t] += add;
+    __syncthreads();
+  }
+  if (t == TRAV_THREADS - 1) {
+    const int total = s_off[t];
+    s_base = total ? atomicAdd(&counters[CNT_QPAIRS], total) : 0;
+  }
+  __syncthreads();
+  const int base = s_base + s_off[t] - nbuf;
+  for (int q = 0; q < nbuf; ++q)
+    if (base + q < qcap) qpairs[base + q] = buf[q * TRAV_THREADS + t];
+  __syncthreads();
+  }
+}
+
+// carry clean pairs / hits / hit-pair records of the previous pass
+constexpr int CARRY_ITEMS = 16;
+__global__ void __launch_bounds__(256) k_carry_pairs(const int2* __restrict__ prev, int n,
+                                                     const uint8_t* __restrict__ dirty_e,
+                                                     int2* __restrict__ cur, int cap,
+                                                     int* __restrict__ counters) {
+  __shared__ int s_off[256];
+  __shared__ int s_base;
+  const int t = threadIdx.x;
+  const int base_i = blockIdx.x * 256 * CARRY_ITEMS;
+  int2 p[CARRY_ITEMS];
+  bool keep[CARRY_ITEMS];
+  int nk = 0;
+#pragma unroll
+  for (int q = 0; q < CARRY_ITEMS; ++q) {
+    const int i = base_i + q * 256 + t;
+    keep[q] = false;
+    if (i < n) {
+      p[q] = prev[i];
+      keep[q] = !dirty_e[p[q].x] && !dirty_e[p[q].y];
+    }
+    nk += keep[q] ? 1 : 0;
+  }
+  s_off[t] = nk;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = (t >= off) ? s_off[t - off] : 0;
+    __syncthreads();
+    s_off[t] += add;
+    __syncthreads();
+  }
+  if (t == 255) s_base = s_off[255] ? atomicAdd(&counters[CNT_PAIRS], s_off[255]) : 0;
+  __syncthreads();
+  int o = s_base + s_off[t] - nk;
+#pragma unroll
+  for (int q = 0; q < CARRY_ITEMS; ++q)
+    if (keep[q]) {
+      if (o < cap) cur[o] = p[q];
+      ++o;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_carry_hits(const hit_rec* __restrict__ prev, int n,
+                                                    const uint8_t* __restrict__ dirty_e,
+                                                    hit_rec* __restrict__ cur, int cap,
+                                                    int* __restrict__ counters) {
+  __shared__ int s_cnt, s_base;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  if (i < n) {
+    const uint64_t key = prev[i].key;
+    const int lo = (int)(key >> 36), hi = (int)((key >> 8) & 0xFFFFFFFull);
+    keep = !dirty_e[lo] && !dirty_e[hi];
+  }
+  const int slot = block_append(keep, &counters[CNT_HITS], &s_cnt, &s_base);
+  if (keep && slot < cap) cur[slot] = prev[i];
+}
+
+__global__ void __launch_bounds__(256) k_carry_hp(const int4* __restrict__ prev, int n,
+                                                  const uint8_t* __restrict__ dirty_e,
+                                                  int4* __restrict__ cur, int cap,
+                                                  int* __restrict__ counters) {
+  __shared__ int s_cnt, s_base;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int4 r = make_int4(0, 0, 0, 0);
+  bool keep = false;
+  if (i < n) {
+    r = prev[i];
+    keep = !dirty_e[r.x] && !dirty_e[r.y];
+  }
+  const int slot = block_append(keep, &counters[CNT_HIT_PAIRS], &s_cnt, &s_base);
+  if (keep) {
+    if (slot < cap) cur[slot] = r;
+    atomicAdd(&counters[CNT_HIT_SUB], __popc(r.z));
+  }
+}
+
+void launch_mark_dirty(hipStream_t st, DevState& S) {
+  (void)hipMemsetAsync(S.tile_dirty, 0, (size_t)S.n_tiles, st);
+  hipLaunchKernelGGL(k_mark_dirty, dim3((S.ne + 255) / 256), dim3(256), 0, st, S.ev, S.dirty_v,
+                     S.ne, S.leaf_of, S.dirty_e, S.tile_dirty, S.dlist, S.counters);
+}
+
+void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
+  // grid sized for every element; lanes beyond the dirty count exit
+  hipLaunchKernelGGL(k_dirty_boxes, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st, S.ev, S.x_old,
+                     S.av, dt, S.dlist, S.counters, S.ebox);
+}
+
+void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
+  hipLaunchKernelGGL(k_query_dirty, dim3(std::min((S.ne + TRAV_THREADS - 1) / TRAV_THREADS, 256)),
+                     dim3(TRAV_THREADS), 0, st, S.nodes, S.leaves, S.ebox, S.ev, S.eflags,
+                     S.dirty_e, S.dlist, S.leaf_of, S.ne, emit, S.qpairs, S.qpair_cap, S.counters);
+}
+
+void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev) {
+  if (npairs_prev > 0)
+    hipLaunchKernelGGL(k_carry_pairs, dim3((npairs_prev + 256 * CARRY_ITEMS - 1) / (256 * CARRY_ITEMS)), dim3(256), 0, st,
+                       S.pairs_prev, npairs_prev, S.dirty_e, S.pairs, S.pair_cap, S.counters);
+  if (nhits_prev > 0)
+    hipLaunchKernelGGL(k_carry_hits, dim3((nhits_prev + 255) / 256), dim3(256), 0, st,
+                       S.hits_prev, nhits_prev, S.dirty_e, S.hits, S.hit_cap, S.counters);
+  if (nhp_prev > 0)
+    hipLaunchKernelGGL(k_carry_hp, dim3((nhp_prev + 255) / 256), dim3(256), 0, st, S.hp_prev,
+                       nhp_prev, S.dirty_e, S.hp, S.hp_cap, S.counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -330,30 +704,46 @@ hipError_t build_bvh(hipStream_t st, DevState& S) {
   hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, S.ebox, n, S.bounds,
                      S.mcode, S.mid);
   bool in_tmp = false;
-  CS_HIP_RET(radix_sort_u32(st, S.mcode, S.mid, S.mcode_tmp, S.mid_tmp, S.sort_hist, n, 30, &in_tmp));
+  CS_HIP_RET(radix_sort_u32(st, S.mcode, S.mid, S.mcode_tmp, S.mid_tmp, S.sort_hist, S.scan_part, n,
+                            30, &in_tmp));
   S.sorted_codes = in_tmp ? S.mcode_tmp : S.mcode;
   S.sorted_ids = in_tmp ? S.mid_tmp : S.mid;
-  if (n > 1)
+  hipLaunchKernelGGL(k_leaf_of, dim3((n + 255) / 256), dim3(256), 0, st, S.sorted_ids, n, S.leaf_of);
+  S.n_top_start = 0;
+  if (n > 1) {
     hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.sorted_codes, n,
-                       S.nodes, S.parent, S.lparent, S.rmax);
+                       S.nodes, S.parent, S.lparent, S.rmin, S.rmax);
+    CS_HIP_RET(hipMemsetAsync(S.counters + CNT_TMP, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_mark_top, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.rmin,
+                       S.rmax, n, S.top_start, S.counters + CNT_TMP);
+    int cnt = 0;
+    CS_HIP_RET(hipMemcpyAsync(&cnt, S.counters + CNT_TMP, sizeof(int), hipMemcpyDeviceToHost, st));
+    CS_HIP_RET(hipStreamSynchronize(st));
+    S.n_top_start = cnt;
+  }
   return hipGetLastError();
 }
 
-hipError_t refit_bvh(hipStream_t st, DevState& S) {
+hipError_t refit_bvh(hipStream_t st, DevState& S, bool only_dirty_tiles) {
   const int n = S.ne;
   if (n == 0) return hipSuccess;
   if (n > 1) CS_HIP_RET(hipMemsetAsync(S.arrive, 0, sizeof(uint32_t) * (size_t)(n - 1), st));
-  hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, st, S.ebox, S.sorted_ids, n,
-                     S.leaves, S.nodes, S.parent, S.lparent, S.arrive);
+  hipLaunchKernelGGL(k_refit_tiles, dim3((n + REFIT_TILE - 1) / REFIT_TILE), dim3(REFIT_THREADS), 0,
+                     st, S.ebox, S.sorted_ids, n, S.leaves, S.nodes, S.parent, S.lparent, S.rmin,
+                     S.rmax, S.arrive, only_dirty_tiles ? S.tile_dirty : nullptr);
+  if (n > 1 && S.n_top_start > 0)
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, S.leaves, S.parent, S.rmin,
+                       S.rmax, S.top_start, S.n_top_start, S.arrive);
   return hipGetLastError();
 }
 
 void launch_traverse(hipStream_t st, DevState& S, int cap) {
   const int n = S.ne;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_traverse, dim3((n + 255) / 256), dim3(256), 0, st, S.nodes, S.leaves,
+  hipLaunchKernelGGL(k_traverse, dim3((n + TRAV_THREADS - 1) / TRAV_THREADS), dim3(TRAV_THREADS), 0,
+                     st, S.nodes, S.leaves,
                      S.rmax, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
-                     S.cand_count, S.err);
+                     (unsigned long long*)(S.counters + CNT_CAND64), S.counters + CNT_ERR);
 }
 
 }  // namespace cs

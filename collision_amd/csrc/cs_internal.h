@@ -23,20 +23,26 @@ enum : int {
   CS_DEVERR_STRAIN = 4,      // strain-limit schedule inconsistency
 };
 
-// slots of the small device counter block
+// slots of the small device counter block (ints; 64-bit counts at 16..21)
 enum {
-  CNT_PAIRS = 0,       // emitted narrow-phase pairs
-  CNT_HITS = 1,        // hit records
-  CNT_HIT_PAIRS = 2,   // pairs with at least one hit
-  CNT_HIT_SUB = 3,     // hit sub-tests
-  CNT_LOG = 4,         // pair-log rows
-  CNT_NARROW = 5,      // pairs whose sub-tests ran
-  CNT_ZONES = 6,       // impact zones updated
-  CNT_ZMEM = 7,        // members of zones with >= 2 points
-  CNT_STRAIN_FLAG = 8, // strain sweep found an edge over the limit
+  CNT_PAIRS = 0,         // pairs in the candidate stream being built
+  CNT_HITS = 1,          // hit records
+  CNT_HIT_PAIRS = 2,     // pairs with at least one hit (hit-pair records)
+  CNT_HIT_SUB = 3,       // hit sub-tests
+  CNT_LOG = 4,           // pair-log rows
+  CNT_TMP = 5,           // scratch count (LBVH top nodes)
+  CNT_ZONES = 6,         // impact zones updated
+  CNT_QPAIRS = 7,        // pairs found by the dirty-element query
+  CNT_STRAIN_FLAG = 8,   // strain sweep state
   CNT_STRAIN_EDGES = 9,
   CNT_STRAIN_DEGEN = 10,
-  CNT_N = 16
+  CNT_STRAIN_SWEEPS = 11,
+  CNT_DIRTY = 12,        // dirty elements
+  CNT_ERR = 15,
+  CNT_CAND64 = 16,       // box-overlap candidates of a full traversal
+  CNT_OLD64 = 18,        // candidates touching dirty elements, previous boxes
+  CNT_NEW64 = 20,        // candidates touching dirty elements, current boxes
+  CNT_INTS = 32
 };
 
 struct NarrowParams {
@@ -66,14 +72,32 @@ struct DevState {
   uint32_t* sort_hist = nullptr;
   size_t sort_hist_words = 0;
   bvh_node *nodes = nullptr, *leaves = nullptr;
-  int32_t *parent = nullptr, *lparent = nullptr, *rmax = nullptr;
+  int32_t *parent = nullptr, *lparent = nullptr, *rmin = nullptr, *rmax = nullptr;
   uint32_t* arrive = nullptr;
-  // candidate stream
+  int32_t* top_start = nullptr;  // nodes spanning several refit tiles whose children do not
+  int n_top_start = 0;           // (built once per LBVH build)
+  uint32_t* scan_part = nullptr; // partial sums of the device scan
+  size_t scan_part_words = 0;
+  int32_t* leaf_of = nullptr;    // element -> sorted leaf index
+  // candidate stream (current and previous pass)
   int2* pairs = nullptr;
+  int2* pairs_prev = nullptr;
   int pair_cap = 0;
-  // hits and contribution records
+  int2* qpairs = nullptr;        // pairs found by the dirty-element query
+  int qpair_cap = 0;
+  // hits (current and previous pass) and hit-pair records (lo, hi, mask, first sub-test)
   hit_rec* hits = nullptr;
+  hit_rec* hits_prev = nullptr;
   int hit_cap = 0;
+  int4* hp = nullptr;
+  int4* hp_prev = nullptr;
+  int hp_cap = 0;
+  // vertices / elements whose average velocity changed since the last narrow phase
+  uint8_t* dirty_v = nullptr;
+  uint8_t* dirty_e = nullptr;
+  int32_t* dlist = nullptr;
+  uint8_t* tile_dirty = nullptr;  // refit tiles holding a dirty leaf
+  int n_tiles = 0;
   uint64_t *hkey = nullptr, *hkey_tmp = nullptr;
   uint32_t *hidx = nullptr, *hidx_tmp = nullptr;
   uint32_t *rkey = nullptr, *rkey_tmp = nullptr, *rval = nullptr, *rval_tmp = nullptr;
@@ -81,33 +105,51 @@ struct DevState {
   // impact zones
   uint32_t *zkey = nullptr, *zkey_tmp = nullptr, *zval = nullptr, *zval_tmp = nullptr;
   int32_t* zsize = nullptr;
+  int32_t* zhead = nullptr;  // first sorted index of each zone
+  int32_t* zend = nullptr;   // one past the last sorted index, by zone root
   // strain-limit schedule (level-ordered element lists)
   int32_t *lvl_ptr = nullptr, *lvl_elems = nullptr;
+  int32_t* elem_level = nullptr;   // level of each element (-1: rigid, not swept)
+  int4* next_level = nullptr;      // per element vertex: level of the vertex's next visit
+  uint8_t* elem_active = nullptr;  // element has an edge over the limit (pre-check)
+  uint8_t* lvl_active = nullptr;   // level holds an active element (pre-check)
+  uint8_t* vdirty = nullptr;       // vertex written earlier in the current sweep
   int n_levels = 0, n_sched = 0;
   // logs
   int4* plog = nullptr;
   int plog_cap = 0;
   // counters / flags
   int* counters = nullptr;
-  unsigned long long* cand_count = nullptr;
-  int* err = nullptr;
 };
 
-// sort
+// sort and scan
 size_t radix_hist_words(int n);
+size_t scan_partial_words(int n);
+hipError_t exclusive_scan_u32(hipStream_t st, uint32_t* data, int n, uint32_t* partials);
 hipError_t radix_sort_u32(hipStream_t st, uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys,
-                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits, bool* result_in_tmp);
+                          uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n, int bits,
+                          bool* result_in_tmp);
 hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys,
-                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits, bool* result_in_tmp);
+                          uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n, int bits,
+                          bool* result_in_tmp);
 
 // broad phase
 void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt);
-hipError_t build_bvh(hipStream_t st, DevState& S);
-hipError_t refit_bvh(hipStream_t st, DevState& S);
+void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt);
+hipError_t build_bvh(hipStream_t st, DevState& S);  // also finds the refit top nodes
+hipError_t refit_bvh(hipStream_t st, DevState& S, bool only_dirty_tiles = false);
+int refit_tile_count(int n);
 void launch_traverse(hipStream_t st, DevState& S, int cap);
+// incremental passes
+void launch_mark_dirty(hipStream_t st, DevState& S);
+void launch_query_dirty(hipStream_t st, DevState& S, int emit);
+void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);
 
 // narrow phase
-void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, int npairs);
+void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
+                   int npairs);
+void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd);
+void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id);
 
 // response
 hipError_t apply_hits(hipStream_t st, DevState& S, int nhits);
@@ -121,6 +163,30 @@ void launch_final_position(hipStream_t st, DevState& S, double dt);
 hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, int* sweeps_out);
 void launch_final_velocity(hipStream_t st, DevState& S);
 void launch_zone_labels(hipStream_t st, DevState& S, int32_t* out);
+
+// Work-group aggregated append: every lane of the block calls it; lanes with
+// want=true get consecutive slots from one global atomic per block.
+// s_cnt / s_base are __shared__ ints of the caller.
+__device__ inline int block_append(bool want, int* counter, int* s_cnt, int* s_base) {
+  if (threadIdx.x == 0) *s_cnt = 0;
+  __syncthreads();
+  int slot = want ? atomicAdd(s_cnt, 1) : -1;
+  __syncthreads();
+  if (threadIdx.x == 0) *s_base = *s_cnt ? atomicAdd(counter, *s_cnt) : 0;
+  __syncthreads();
+  return want ? *s_base + slot : -1;
+}
+
+__device__ inline unsigned long long block_sum_u64(unsigned long long v, unsigned long long* s_w) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += s_w[w];
+  __syncthreads();
+  return t;  // valid in thread 0
+}
 
 // float atomics via CAS (monotone min/max on finite values)
 __device__ inline void atomicMin_float(float* addr, float v) {

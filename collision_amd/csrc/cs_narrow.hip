@@ -408,8 +408,9 @@ __global__ void __launch_bounds__(NARROW_THREADS)
     k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
              const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
              const uint8_t* __restrict__ fixed, NarrowParams P, int* __restrict__ uf,
-             hit_rec* __restrict__ hits, int hit_cap, int* __restrict__ counters,
-             int4* __restrict__ plog, int plog_cap, int* __restrict__ err) {
+             hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
+             int* __restrict__ counters) {
+  int* err = counters + CNT_ERR;
   __shared__ double lx[6 * 3 * NARROW_THREADS];
   __shared__ double lv[6 * 3 * NARROW_THREADS];
   __shared__ uint8_t lst[6 * NARROW_THREADS];
@@ -480,12 +481,9 @@ __global__ void __launch_bounds__(NARROW_THREADS)
     }
   }
   if (mask) {
-    atomicAdd(&counters[CNT_HIT_PAIRS], 1);
+    const int slot = atomicAdd(&counters[CNT_HIT_PAIRS], 1);
+    if (slot < hp_cap) hp[slot] = make_int4(B, A, mask, first_hit);
     atomicAdd(&counters[CNT_HIT_SUB], __popc(mask));
-    if (P.log_pairs) {
-      const int slot = atomicAdd(&counters[CNT_LOG], 1);
-      if (slot < plog_cap) plog[slot] = make_int4(P.pass_id, B, A, mask);
-    }
     if (P.zone_mode) {
       for (int s = first_hit; s < nsub; ++s) {
         int q[4];
@@ -500,11 +498,65 @@ __global__ void __launch_bounds__(NARROW_THREADS)
   }
 }
 
-void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, int npairs) {
+void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
+                   int npairs) {
   if (npairs <= 0) return;
   hipLaunchKernelGGL(k_narrow, dim3((npairs + NARROW_THREADS - 1) / NARROW_THREADS),
-                     dim3(NARROW_THREADS), 0, st, S.pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
-                     S.uf, S.hits, S.hit_cap, S.counters, S.plog, S.plog_cap, S.err);
+                     dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
+                     S.uf, S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
+}
+
+// impact-zone merges of hit pairs carried from the previous pass: the points
+// of every sub-test from the pair's first hit onward
+__global__ void k_hp_unions(const int4* __restrict__ hp, int n, const int4* __restrict__ ev,
+                            int ccd, int* __restrict__ uf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 r = hp[i];
+  const int A = r.y, B = r.x;
+  const int4 ta = ev[A], tb = ev[B];
+  const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
+  int kind, nsub, g[6];
+  if (tri_a && tri_b) {
+    kind = 0; nsub = 15;
+    g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
+  } else if (tri_a || tri_b) {
+    const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
+    kind = 1; nsub = 5;
+    g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
+  } else {
+    kind = 2; nsub = 1;
+    g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
+  }
+  for (int s = r.w; s < nsub; ++s) {
+    int q[4];
+    bool is_pt;
+    subtest_points(kind, s, ccd, q, &is_pt);
+    uf_union(uf, g[q[0]], g[q[1]]);
+    uf_union(uf, g[q[0]], g[q[2]]);
+    uf_union(uf, g[q[0]], g[q[3]]);
+  }
+}
+
+void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, S.ev, ccd, S.uf);
+}
+
+// parity log: (pass, lo, hi, mask) for every hit pair of the pass
+__global__ void k_hp_log(const int4* __restrict__ hp, int n, int pass_id, int4* __restrict__ plog,
+                         int cap, int* __restrict__ counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int4 r = hp[i];
+  const int slot = atomicAdd(&counters[CNT_LOG], 1);
+  if (slot < cap) plog[slot] = make_int4(pass_id, r.x, r.y, r.z);
+}
+
+void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hp_log, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, pass_id, S.plog,
+                     S.plog_cap, S.counters);
 }
 
 }  // namespace cs

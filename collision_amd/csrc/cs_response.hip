@@ -19,6 +19,8 @@
 //    original order, so a level-by-level sweep reproduces the sequential
 //    one exactly.  A parallel pre-check skips sweeps in which no edge is over
 //    the limit (such a sweep changes nothing and ends the loop).
+#include <algorithm>
+
 #include "cs_internal.h"
 
 namespace cs {
@@ -89,7 +91,7 @@ __global__ void k_expand_records(const hit_rec* __restrict__ hits,
 __global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __restrict__ rval,
                         int nrec, int nv, const hit_rec* __restrict__ hits,
                         dvec4* __restrict__ av, uint8_t* __restrict__ has,
-                        int* __restrict__ err) {
+                        uint8_t* __restrict__ dirty_v, int* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrec) return;
   const uint32_t v = rkey[i];
@@ -115,6 +117,7 @@ __global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __res
   if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(err, CS_DEVERR_NONFINITE);
   av[v] = a;
   has[v] = 1;
+  dirty_v[v] = 1;
 }
 
 static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rounded to 8
@@ -130,18 +133,18 @@ hipError_t apply_hits(hipStream_t st, DevState& S, int nhits) {
   hipLaunchKernelGGL(k_hit_keys, dim3(nb), dim3(256), 0, st, S.hits, nhits, S.hkey, S.hidx);
   bool in_tmp = false;
   const int kbits = bits_for(hit_key((uint32_t)S.ne, (uint32_t)S.ne, 255u));
-  CS_HIP_RET(radix_sort_u64(st, S.hkey, S.hidx, S.hkey_tmp, S.hidx_tmp, S.sort_hist, nhits, kbits,
+  CS_HIP_RET(radix_sort_u64(st, S.hkey, S.hidx, S.hkey_tmp, S.hidx_tmp, S.sort_hist, S.scan_part, nhits, kbits,
                             &in_tmp));
   const uint32_t* sidx = in_tmp ? S.hidx_tmp : S.hidx;
   const int nrec = 4 * nhits;
   hipLaunchKernelGGL(k_expand_records, dim3((nrec + 255) / 256), dim3(256), 0, st, S.hits, sidx,
                      nhits, S.nv, S.rkey, S.rval);
-  CS_HIP_RET(radix_sort_u32(st, S.rkey, S.rval, S.rkey_tmp, S.rval_tmp, S.sort_hist, nrec,
+  CS_HIP_RET(radix_sort_u32(st, S.rkey, S.rval, S.rkey_tmp, S.rval_tmp, S.sort_hist, S.scan_part, nrec,
                             bits_for((uint64_t)S.nv), &in_tmp));
   const uint32_t* rk = in_tmp ? S.rkey_tmp : S.rkey;
   const uint32_t* rv = in_tmp ? S.rval_tmp : S.rval;
   hipLaunchKernelGGL(k_apply, dim3((nrec + 255) / 256), dim3(256), 0, st, rk, rv, nrec, S.nv,
-                     S.hits, S.av, S.has, S.err);
+                     S.hits, S.av, S.has, S.dirty_v, S.counters + CNT_ERR);
   return hipGetLastError();
 }
 
@@ -174,110 +177,146 @@ __global__ void k_zone_keys(const uint32_t* __restrict__ root_of, const int32_t*
   zval[v] = (uint32_t)v;
 }
 
-// updateImpactListVelocity (dcollid3d.cpp:70-200), members ascending
-__global__ void k_zone_update(const uint32_t* __restrict__ zkey, const uint32_t* __restrict__ zval,
-                              int nv, const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
-                              const uint8_t* __restrict__ fixed, double m, double dt,
-                              int* __restrict__ counters, int* __restrict__ err) {
+// zone heads (first sorted index) and ends (by root)
+__global__ void __launch_bounds__(256) k_zone_heads(const uint32_t* __restrict__ zkey, int nv,
+                                                   int32_t* __restrict__ zhead,
+                                                   int32_t* __restrict__ zend,
+                                                   int* __restrict__ counters) {
+  __shared__ int s_cnt, s_base;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const uint32_t z = zkey[i];
-  if (z >= (uint32_t)nv) return;
-  if (i > 0 && zkey[i - 1] == z) return;
-  int end = i;
-  while (end < nv && zkey[end] == z) ++end;
-  atomicAdd(&counters[CNT_ZONES], 1);
-  double x_cm[3] = {0, 0, 0}, v_cm[3] = {0, 0, 0}, L[3] = {0, 0, 0};
-  double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tmp[3][3];
-  int num_pts = 0;
-  for (int j = i; j < end; ++j) {
-    const int p = zval[j];
-    const dvec4 o = x_old[p], a = av[p];
-    num_pts++;
-    x_cm[0] += o.x; x_cm[1] += o.y; x_cm[2] += o.z;
-    v_cm[0] += a.x; v_cm[1] += a.y; v_cm[2] += a.z;
+  bool head = false;
+  if (i < nv) {
+    const uint32_t z = zkey[i];
+    if (z < (uint32_t)nv) {
+      head = (i == 0) || zkey[i - 1] != z;
+      if (i == nv - 1 || zkey[i + 1] != z) zend[z] = i + 1;
+    }
   }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    x_cm[k] /= num_pts;
-    v_cm[k] /= num_pts;
-  }
-  for (int j = i; j < end; ++j) {
-    const int p = zval[j];
-    const dvec4 o = x_old[p], a = av[p];
-    const double xo[3] = {o.x, o.y, o.z}, vv[3] = {a.x, a.y, a.z};
-    double dx[3], dv[3], Li[3];
-    sub3(xo, x_cm, dx);
-    sub3(vv, v_cm, dv);
-    cross3(dx, dv, Li);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) Li[k] = m * Li[k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) L[k] = Li[k] + L[k];
-  }
-  for (int j = i; j < end; ++j) {
-    const int p = zval[j];
-    const dvec4 o = x_old[p];
-    const double xo[3] = {o.x, o.y, o.z};
-    double dx[3];
-    sub3(xo, x_cm, dx);
-    const double mag_dx = mag3(dx);
-#pragma unroll
+  const int slot = block_append(head, &counters[CNT_ZONES], &s_cnt, &s_base);
+  if (head) zhead[slot] = i;
+}
+
+// updateImpactListVelocity (dcollid3d.cpp:70-200), one 64-lane wavefront per
+// zone.  Sums run in the canonical lane order: lane l adds the terms of
+// members l, l+64, ... (ascending vertex id) from 0.0, then the 64 lane sums
+// are added in lane order.
+constexpr int ZONE_LANES = 64;
+
+__device__ inline void lane_total(double (*part)[9], int nout, double* tot) {
+  for (int k = 0; k < nout; ++k) tot[k] = 0.0;
+  for (int l = 0; l < ZONE_LANES; ++l)
+    for (int k = 0; k < nout; ++k) tot[k] = tot[k] + part[l][k];
+}
+
+__global__ void __launch_bounds__(ZONE_LANES)
+    k_zone_update(const uint32_t* __restrict__ zkey, const uint32_t* __restrict__ zval,
+                  const int32_t* __restrict__ zhead, const int32_t* __restrict__ zend,
+                  const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
+                  const uint8_t* __restrict__ fixed, double m, double dt,
+                  int* __restrict__ counters, uint8_t* __restrict__ dirty_v) {
+  __shared__ double part[ZONE_LANES][9];
+  int* err = counters + CNT_ERR;
+  const int l = threadIdx.x;
+  const int nz = counters[CNT_ZONES];
+  for (int z = blockIdx.x; z < nz; z += gridDim.x) {
+    const int b = zhead[z];
+    const int e = zend[zkey[b]];
+    const int n = e - b;
+    // x_cm, v_cm
+    double t[9];
+    for (int k = 0; k < 6; ++k) t[k] = 0.0;
+    for (int j = b + l; j < e; j += ZONE_LANES) {
+      const int p = zval[j];
+      const dvec4 o = x_old[p], a = av[p];
+      t[0] = o.x + t[0]; t[1] = o.y + t[1]; t[2] = o.z + t[2];
+      t[3] = a.x + t[3]; t[4] = a.y + t[4]; t[5] = a.z + t[5];
+    }
+    for (int k = 0; k < 6; ++k) part[l][k] = t[k];
+    __syncthreads();
+    double tot[9];
+    lane_total(part, 6, tot);
+    const double x_cm[3] = {tot[0] / n, tot[1] / n, tot[2] / n};
+    const double v_cm[3] = {tot[3] / n, tot[4] / n, tot[5] / n};
+    __syncthreads();
+    // angular momentum L = sum m dx x dv
+    for (int k = 0; k < 3; ++k) t[k] = 0.0;
+    for (int j = b + l; j < e; j += ZONE_LANES) {
+      const int p = zval[j];
+      const dvec4 o = x_old[p], a = av[p];
+      const double xo[3] = {o.x, o.y, o.z}, vv[3] = {a.x, a.y, a.z};
+      double dx[3], dv[3], Li[3];
+      sub3(xo, x_cm, dx);
+      sub3(vv, v_cm, dv);
+      cross3(dx, dv, Li);
+      for (int k = 0; k < 3; ++k) t[k] = m * Li[k] + t[k];
+    }
+    for (int k = 0; k < 3; ++k) part[l][k] = t[k];
+    __syncthreads();
+    double L[3];
+    lane_total(part, 3, L);
+    __syncthreads();
+    // inertia tensor I = sum m (|dx|^2 delta - dx dx^T)
+    for (int k = 0; k < 9; ++k) t[k] = 0.0;
+    for (int j = b + l; j < e; j += ZONE_LANES) {
+      const int p = zval[j];
+      const dvec4 o = x_old[p];
+      const double xo[3] = {o.x, o.y, o.z};
+      double dx[3];
+      sub3(xo, x_cm, dx);
+      const double mag_dx = mag3(dx);
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double tm = -dx[r] * dx[c];
+          if (r == c) tm += mag_dx * mag_dx;
+          t[3 * r + c] = tm * m + t[3 * r + c];
+        }
+    }
+    for (int k = 0; k < 9; ++k) part[l][k] = t[k];
+    __syncthreads();
+    double Iv[9];
+    lane_total(part, 9, Iv);
+    __syncthreads();
+    double I[3][3], tmp[3][3];
     for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        tmp[r][c] = -dx[r] * dx[c];
-        if (r == c) tmp[r][c] += mag_dx * mag_dx;
-        I[r][c] += tmp[r][c] * m;
+      for (int c = 0; c < 3; ++c) I[r][c] = Iv[3 * r + c];
+    double w[3];
+    for (int c = 0; c < 3; ++c) {
+      for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) tmp[r][q] = I[r][q];
+      for (int r = 0; r < 3; ++r) tmp[r][c] = L[r];
+      if (det3(I) < CS_ROUND_EPS) w[c] = 0.0;
+      else w[c] = det3(tmp) / det3(I);
+    }
+    const double mag_w = mag3(w);
+    for (int j = b + l; j < e; j += ZONE_LANES) {
+      const int p = zval[j];
+      if (fixed[p]) continue;
+      const dvec4 o = x_old[p];
+      const double xo[3] = {o.x, o.y, o.z};
+      double dx[3], xF[3], xR[3], wxR[3], tmpV[3], x_new[3], an[3];
+      sub3(xo, x_cm, dx);
+      if (mag_w < CS_ROUND_EPS) {
+        for (int k = 0; k < 3; ++k) {
+          xF[k] = dx[k];
+          wxR[k] = 0.0;
+        }
+        sub3(dx, xF, xR);
+      } else {
+        const double sc = dot3(dx, w) / dot3(w, w);
+        for (int k = 0; k < 3; ++k) xF[k] = sc * w[k];
+        sub3(dx, xF, xR);
+        const double sw = cs_det_sin(dt * mag_w) / mag_w;
+        for (int k = 0; k < 3; ++k) tmpV[k] = sw * w[k];
+        cross3(tmpV, xR, wxR);
       }
-  }
-  double w[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) tmp[r][q] = I[r][q];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) tmp[r][c] = L[r];
-    if (det3(I) < CS_ROUND_EPS) w[c] = 0.0;
-    else w[c] = det3(tmp) / det3(I);
-  }
-  const double mag_w = mag3(w);
-  for (int j = i; j < end; ++j) {
-    const int p = zval[j];
-    if (fixed[p]) continue;
-    const dvec4 o = x_old[p];
-    const double xo[3] = {o.x, o.y, o.z};
-    double dx[3], xF[3], xR[3], wxR[3], tmpV[3], x_new[3];
-    sub3(xo, x_cm, dx);
-    if (mag_w < CS_ROUND_EPS) {
-#pragma unroll
       for (int k = 0; k < 3; ++k) {
-        xF[k] = dx[k];
-        wxR[k] = 0.0;
+        x_new[k] = x_cm[k] + dt * v_cm[k] + xF[k] + cs_det_cos(dt * mag_w) * xR[k] + wxR[k];
+        an[k] = (x_new[k] - xo[k]) / dt;
       }
-      sub3(dx, xF, xR);
-    } else {
-      const double sc = dot3(dx, w) / dot3(w, w);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) xF[k] = sc * w[k];
-      sub3(dx, xF, xR);
-      const double sw = cs_det_sin(dt * mag_w) / mag_w;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) tmpV[k] = sw * w[k];
-      cross3(tmpV, xR, wxR);
+      if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
+      av[p] = dvec4{an[0], an[1], an[2], 0.0};
+      dirty_v[p] = 1;
     }
-    dvec4 a;
-    double an[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      x_new[k] = x_cm[k] + dt * v_cm[k] + xF[k] + cs_det_cos(dt * mag_w) * xR[k] + wxR[k];
-      an[k] = (x_new[k] - xo[k]) / dt;
-    }
-    if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
-    a.x = an[0]; a.y = an[1]; a.z = an[2]; a.w = 0.0;
-    av[p] = a;
   }
 }
 
@@ -289,12 +328,13 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt) {
   hipLaunchKernelGGL(k_zone_keys, dim3(nb), dim3(256), 0, st, S.zval_tmp, S.zsize, nv, S.zkey,
                      S.zval);
   bool in_tmp = false;
-  CS_HIP_RET(radix_sort_u32(st, S.zkey, S.zval, S.zkey_tmp, S.zval_tmp, S.sort_hist, nv,
+  CS_HIP_RET(radix_sort_u32(st, S.zkey, S.zval, S.zkey_tmp, S.zval_tmp, S.sort_hist, S.scan_part, nv,
                             bits_for((uint64_t)nv), &in_tmp));
   const uint32_t* zk = in_tmp ? S.zkey_tmp : S.zkey;
   const uint32_t* zv = in_tmp ? S.zval_tmp : S.zval;
-  hipLaunchKernelGGL(k_zone_update, dim3(nb), dim3(256), 0, st, zk, zv, nv, S.x_old, S.av, S.fixed,
-                     m, dt, S.counters, S.err);
+  hipLaunchKernelGGL(k_zone_heads, dim3(nb), dim3(256), 0, st, zk, nv, S.zhead, S.zend, S.counters);
+  hipLaunchKernelGGL(k_zone_update, dim3(std::min(nv, 2048)), dim3(ZONE_LANES), 0, st, zk, zv,
+                     S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters, S.dirty_v);
   return hipGetLastError();
 }
 
@@ -390,7 +430,7 @@ void launch_predict(hipStream_t st, DevState& S, double dt) {
 
 void launch_average_velocity(hipStream_t st, DevState& S, double dt) {
   hipLaunchKernelGGL(k_average_velocity, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old,
-                     S.av, dt, S.nv, S.err);
+                     S.av, dt, S.nv, S.counters + CNT_ERR);
 }
 
 void launch_uf_init(hipStream_t st, DevState& S) {
@@ -416,11 +456,20 @@ __device__ inline bool edge_over_limit(const dvec4& o0, const dvec4& o1, const d
   return true;
 }
 
-// parallel check: does any edge exceed the limit with the current velocities?
+// Sparse exact sweep.  An element none of whose vertices was written earlier
+// in the sweep reads the pre-sweep velocities, so whether it writes is
+// decided by the parallel pre-check (elem_active).  Only pre-check-active
+// elements and later visits of written ("dirty") vertices are evaluated, in
+// level order; a per-level flag in LDS (set by the pre-check and, on every
+// dirty vertex, for the level of the vertex's next visit) lets the single
+// work-group skip idle levels without touching memory.
+
+// pre-check with the current velocities: elem_active / lvl_active / any flag
 __global__ void k_strain_check(const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
                                const dvec4* __restrict__ len0, const dvec4* __restrict__ x_old,
-                               const dvec4* __restrict__ av, double dt, int ne,
-                               int* __restrict__ counters) {
+                               const dvec4* __restrict__ av, const int32_t* __restrict__ elem_level,
+                               double dt, int ne, uint8_t* __restrict__ elem_active,
+                               uint8_t* __restrict__ lvl_active, int* __restrict__ counters) {
   if (counters[CNT_STRAIN_FLAG] < 0) return;  // loop already finished
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool any = false;
@@ -435,17 +484,22 @@ __global__ void k_strain_check(const int4* __restrict__ ev, const uint8_t* __res
       bool deg;
       if (edge_over_limit(x_old[p0], x_old[p1], av[p0], av[p1], L0[j], dt, &deg)) any = true;
     }
+    elem_active[e] = any ? 1 : 0;
+    if (any) lvl_active[elem_level[e]] = 1;
   }
   if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(&counters[CNT_STRAIN_FLAG], 1);
 }
 
-// one exact sequential sweep, level by level, by a single work-group
 constexpr int STRAIN_THREADS = 1024;
+constexpr int STRAIN_LDS_LEVELS = 65536;  // levels whose flags fit in LDS
 __global__ void __launch_bounds__(STRAIN_THREADS)
     k_strain_sweep(const int32_t* __restrict__ lvl_ptr, const int32_t* __restrict__ lvl_elems,
                    int n_levels, const int4* __restrict__ ev, const dvec4* __restrict__ len0,
+                   const int4* __restrict__ next_level, const uint8_t* __restrict__ elem_active,
+                   const uint8_t* __restrict__ lvl_active_g, uint8_t* vdirty,
                    const dvec4* __restrict__ x_old, dvec4* av, double dt,
                    int* __restrict__ counters) {
+  __shared__ uint8_t act[STRAIN_LDS_LEVELS];
   __shared__ int s_flag, s_edges, s_degen, s_state;
   if (threadIdx.x == 0) {
     s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweep needed
@@ -460,21 +514,29 @@ __global__ void __launch_bounds__(STRAIN_THREADS)
     if (threadIdx.x == 0) {
       counters[CNT_STRAIN_FLAG] = -1;
       counters[CNT_STRAIN_EDGES] = 0;
-      atomicAdd(&counters[CNT_STRAIN_DEGEN + 1], 1);  // sweeps executed
+      atomicAdd(&counters[CNT_STRAIN_SWEEPS], 1);
     }
     return;
   }
+  const bool use_lds = n_levels <= STRAIN_LDS_LEVELS;
+  if (use_lds)
+    for (int L = threadIdx.x; L < n_levels; L += STRAIN_THREADS) act[L] = lvl_active_g[L];
+  __syncthreads();
   for (int L = 0; L < n_levels; ++L) {
+    if (use_lds && !act[L]) continue;  // uniform: flags change only before a barrier
     const int b = lvl_ptr[L], e_end = lvl_ptr[L + 1];
     for (int idx = b + threadIdx.x; idx < e_end; idx += STRAIN_THREADS) {
       const int e = lvl_elems[idx];
       const int4 t = ev[e];
       const int np = (t.z >= 0) ? 3 : 2;
       const int vid[3] = {t.x, t.y, t.z};
+      bool dirty[3] = {vdirty[t.x] != 0, vdirty[t.y] != 0, np == 3 && vdirty[t.z] != 0};
+      if (!elem_active[e] && !dirty[0] && !dirty[1] && !dirty[2]) continue;
       const dvec4 l0 = len0[e];
       const double L0[3] = {l0.x, l0.y, l0.z};
       for (int j = 0; j < ((np == 2) ? 1 : np); ++j) {
-        const int p0 = vid[j % np], p1 = vid[(j + 1) % np];
+        const int s0 = j % np, s1 = (j + 1) % np;
+        const int p0 = vid[s0], p1 = vid[s1];
         const dvec4 a0 = av[p0], a1 = av[p1];
         bool deg;
         if (edge_over_limit(x_old[p0], x_old[p1], a0, a1, L0[j], dt, &deg)) {
@@ -485,10 +547,19 @@ __global__ void __launch_bounds__(STRAIN_THREADS)
           const dvec4 nv4 = dvec4{vt[0], vt[1], vt[2], 0.0};
           av[p0] = nv4;
           av[p1] = nv4;
+          dirty[s0] = dirty[s1] = true;
           s_flag = 1;
           atomicAdd(&s_edges, 1);
           if (deg) atomicAdd(&s_degen, 1);
         }
+      }
+      // every later visit of a dirty vertex must be evaluated
+      const int4 nl = next_level[e];
+      const int nls[3] = {nl.x, nl.y, nl.z};
+      for (int q = 0; q < np; ++q) {
+        if (!dirty[q]) continue;
+        vdirty[vid[q]] = 1;
+        if (use_lds && nls[q] >= 0) act[nls[q]] = 1;
       }
     }
     __syncthreads();
@@ -496,7 +567,7 @@ __global__ void __launch_bounds__(STRAIN_THREADS)
   if (threadIdx.x == 0) {
     counters[CNT_STRAIN_EDGES] = s_edges;
     counters[CNT_STRAIN_DEGEN] += s_degen;
-    counters[CNT_STRAIN_DEGEN + 1] += 1;  // sweeps executed
+    counters[CNT_STRAIN_SWEEPS] += 1;
     counters[CNT_STRAIN_FLAG] = s_flag ? 0 : -1;  // next check decides; no edge -> finished
   }
 }
@@ -506,10 +577,14 @@ hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, 
   const int ne = S.ne;
   if (ne == 0 || S.n_sched == 0 || max_sweeps <= 0) return hipSuccess;
   for (int s = 0; s < max_sweeps; ++s) {
+    CS_HIP_RET(hipMemsetAsync(S.lvl_active, 0, (size_t)S.n_levels, st));
+    CS_HIP_RET(hipMemsetAsync(S.vdirty, 0, (size_t)S.nv, st));
     hipLaunchKernelGGL(k_strain_check, dim3((ne + 255) / 256), dim3(256), 0, st, S.ev, S.eflags,
-                       S.len0, S.x_old, S.av, dt, ne, S.counters);
+                       S.len0, S.x_old, S.av, S.elem_level, dt, ne, S.elem_active, S.lvl_active,
+                       S.counters);
     hipLaunchKernelGGL(k_strain_sweep, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
-                       S.lvl_elems, S.n_levels, S.ev, S.len0, S.x_old, S.av, dt, S.counters);
+                       S.lvl_elems, S.n_levels, S.ev, S.len0, S.next_level, S.elem_active,
+                       S.lvl_active, S.vdirty, S.x_old, S.av, dt, S.counters);
   }
   return hipGetLastError();
 }

@@ -3,7 +3,7 @@
 // One pass per 8-bit digit, three kernels per pass (reduce-then-scan):
 //   1. k_digit_hist  : per-tile 256-bin histogram (LDS atomics), written
 //                      digit-major so a plain exclusive scan gives offsets.
-//   2. k_scan_single : one work-group exclusive scan of the histogram table.
+//   2. exclusive_scan_u32 : device-wide scan of the histogram table.
 //   3. k_scatter     : per-tile stable ranking.  Each 64-lane wavefront owns a
 //                      contiguous quarter of the tile and walks it in 64-key
 //                      rounds; lanes holding the same digit are found with 8
@@ -40,29 +40,104 @@ __global__ void __launch_bounds__(SORT_THREADS) k_digit_hist(const K* __restrict
   for (int i = threadIdx.x; i < 256; i += SORT_THREADS) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
 }
 
-// Exclusive scan of n uint32 values in place by one work-group of 1024.
-__global__ void __launch_bounds__(1024) k_scan_single(uint32_t* __restrict__ data, int n) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int b = t * chunk;
-  const int e = min(n, b + chunk);
+// Device-wide exclusive scan of uint32 in three coalesced phases:
+// per-tile sums, one work-group scanning the tile sums, per-tile scan + offset.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const uint32_t* __restrict__ data,
+                                                              int n, uint32_t* __restrict__ part) {
+  __shared__ uint32_t red[SCAN_THREADS];
+  const int base = blockIdx.x * SCAN_TILE;
   uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += data[i];
-  part[t] = s;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int idx = base + k * SCAN_THREADS + threadIdx.x;
+    if (idx < n) s += data[idx];
+  }
+  red[threadIdx.x] = s;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = (t >= off) ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
+  for (int off = SCAN_THREADS / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
     __syncthreads();
   }
-  uint32_t run = part[t] - s;
-  for (int i = b; i < e; ++i) {
-    uint32_t v = data[i];
-    data[i] = run;
-    run += v;
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// exclusive scan of part[0..np) by one work-group, in coalesced 1024-wide tiles
+__global__ void __launch_bounds__(1024) k_scan_partials(uint32_t* __restrict__ part, int np) {
+  __shared__ uint32_t buf[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < np; base += 1024) {
+    const int idx = base + threadIdx.x;
+    const uint32_t v = idx < np ? part[idx] : 0u;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t add = ((int)threadIdx.x >= off) ? buf[threadIdx.x - off] : 0u;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const uint32_t incl = buf[threadIdx.x];
+    const uint32_t c = carry;
+    if (idx < np) part[idx] = c + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = c + incl;
+    __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_down(uint32_t* __restrict__ data, int n,
+                                                            const uint32_t* __restrict__ part) {
+  __shared__ uint32_t tile[SCAN_TILE];
+  __shared__ uint32_t tsum[SCAN_THREADS];
+  const int base = blockIdx.x * SCAN_TILE;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int idx = base + k * SCAN_THREADS + t;
+    tile[k * SCAN_THREADS + t] = idx < n ? data[idx] : 0u;
+  }
+  __syncthreads();
+  uint32_t loc[SCAN_ITEMS];
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    loc[k] = run;
+    run += tile[t * SCAN_ITEMS + k];
+  }
+  tsum[t] = run;
+  __syncthreads();
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+    const uint32_t add = (t >= off) ? tsum[t - off] : 0u;
+    __syncthreads();
+    tsum[t] += add;
+    __syncthreads();
+  }
+  const uint32_t pre = part[blockIdx.x] + tsum[t] - run;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) tile[t * SCAN_ITEMS + k] = pre + loc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    const int idx = base + k * SCAN_THREADS + t;
+    if (idx < n) data[idx] = tile[k * SCAN_THREADS + t];
+  }
+}
+
+size_t scan_partial_words(int n) { return (size_t)((n + SCAN_TILE - 1) / SCAN_TILE) + 1; }
+
+hipError_t exclusive_scan_u32(hipStream_t st, uint32_t* data, int n, uint32_t* partials) {
+  if (n <= 0) return hipSuccess;
+  const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nt), dim3(SCAN_THREADS), 0, st, data, n, partials);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, st, partials, nt);
+  hipLaunchKernelGGL(k_scan_down, dim3(nt), dim3(SCAN_THREADS), 0, st, data, n, partials);
+  return hipGetLastError();
 }
 
 template <typename K>
@@ -134,8 +209,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_scatter(const K* __restrict__ 
 
 template <typename K>
 static hipError_t radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, K* tmp_keys,
-                                  uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
-                                  bool* result_in_tmp) {
+                                  uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n,
+                                  int bits, bool* result_in_tmp) {
   *result_in_tmp = false;
   if (n <= 1 || bits <= 0) return hipSuccess;
   const int nblocks = (n + SORT_TILE - 1) / SORT_TILE;
@@ -146,7 +221,7 @@ static hipError_t radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, K* tm
   for (int shift = 0; shift < bits; shift += 8) {
     hipLaunchKernelGGL(k_digit_hist<K>, dim3(nblocks), dim3(SORT_THREADS), 0, st, kin, n, shift,
                        hist, nblocks);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, hist, 256 * nblocks);
+    CS_HIP_RET(exclusive_scan_u32(st, hist, 256 * nblocks, partials));
     hipLaunchKernelGGL(k_scatter<K>, dim3(nblocks), dim3(SORT_THREADS), 0, st, kin, vin, kout,
                        vout, n, shift, hist, nblocks);
     K* tk = kin; kin = kout; kout = tk;
@@ -162,16 +237,16 @@ size_t radix_hist_words(int n) {
 }
 
 hipError_t radix_sort_u32(hipStream_t st, uint32_t* keys, uint32_t* vals, uint32_t* tmp_keys,
-                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
+                          uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n, int bits,
                           bool* result_in_tmp) {
-  return radix_sort_impl<uint32_t>(st, keys, vals, tmp_keys, tmp_vals, hist, n, bits,
+  return radix_sort_impl<uint32_t>(st, keys, vals, tmp_keys, tmp_vals, hist, partials, n, bits,
                                    result_in_tmp);
 }
 
 hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64_t* tmp_keys,
-                          uint32_t* tmp_vals, uint32_t* hist, int n, int bits,
+                          uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n, int bits,
                           bool* result_in_tmp) {
-  return radix_sort_impl<uint64_t>(st, keys, vals, tmp_keys, tmp_vals, hist, n, bits,
+  return radix_sort_impl<uint64_t>(st, keys, vals, tmp_keys, tmp_vals, hist, partials, n, bits,
                                    result_in_tmp);
 }
 

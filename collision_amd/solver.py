@@ -172,7 +172,8 @@ class CollisionSolver3d:
         return [dict(kind=_lib.PASS_NAMES[p.kind], index=p.index, candidates=p.candidates,
                      narrow_pairs=p.narrow_pairs, hit_pairs=p.hit_pairs,
                      hit_subtests=p.hit_subtests, ms_broad=p.ms_broad, ms_narrow=p.ms_narrow,
-                     ms_response=p.ms_response) for p in arr[:n]]
+                     ms_response=p.ms_response, ms_traverse_kernel=p.ms_traverse_kernel,
+                     ms_narrow_kernel=p.ms_narrow_kernel) for p in arr[:n]]
 
     def pairLog(self) -> np.ndarray:
         n = self.L.cs_pair_log_count(self.h)

@@ -56,7 +56,11 @@ typedef struct cs_params {
     int32_t max_strain_sweeps;  /* 10 (dcollid.cpp:590) */
     int32_t log_pairs;          /* record hit pairs of every pass (parity taps) */
     int32_t device;             /* HIP device ordinal */
-    int32_t reserved[3];
+    int32_t force_full_passes;  /* 1: every pass re-runs the whole broad and narrow
+                                   phase (0: passes after the first CCD pass of a
+                                   step only redo pairs whose vertices changed;
+                                   results are identical either way) */
+    int32_t reserved[2];
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
@@ -79,7 +83,9 @@ typedef struct cs_pass_info {
     int64_t narrow_pairs;    /* pairs whose sub-tests ran */
     int64_t hit_pairs;       /* pairs whose narrow phase reported contact */
     int64_t hit_subtests;
-    double ms_broad, ms_narrow, ms_response;
+    double ms_broad, ms_narrow, ms_response;  /* phase wall time on the handle's stream */
+    double ms_traverse_kernel;                /* LBVH traversal kernel alone */
+    double ms_narrow_kernel;                  /* narrow-phase kernel alone */
 } cs_pass_info;
 
 typedef struct cs_step_report {

@@ -993,41 +993,87 @@ static void update_average_velocity(orc_solver *s) {
     }
 }
 
-/* updateImpactListVelocity (dcollid3d.cpp:70-200) on members[0..n) */
+/* Sum of per-member terms in one of two orders:
+ *   lanes == 0  : sequential over members (the reference's linked-list walk)
+ *   lanes == 64 : canonical order -- lane l sums members l, l+64, l+128, ...
+ *                 from 0.0, then the 64 lane sums are added in lane order.
+ * The canonical order is the one a 64-wide wavefront computes per zone. */
+typedef void (*term_fn)(const void *ctx, int32_t p, double *out, int nout);
+
+static void ordered_sum(int lanes, int32_t n, const int32_t *mem, term_fn f, const void *ctx,
+                        int nout, double *total) {
+    double t[9];
+    for (int k = 0; k < nout; ++k) total[k] = 0.0;
+    if (lanes == 0) {
+        for (int32_t q = 0; q < n; ++q) {
+            f(ctx, mem[q], t, nout);
+            for (int k = 0; k < nout; ++k) total[k] = t[k] + total[k];
+        }
+        return;
+    }
+    for (int l = 0; l < lanes; ++l) {
+        double part[9];
+        for (int k = 0; k < nout; ++k) part[k] = 0.0;
+        for (int32_t q = l; q < n; q += lanes) {
+            f(ctx, mem[q], t, nout);
+            for (int k = 0; k < nout; ++k) part[k] = t[k] + part[k];
+        }
+        for (int k = 0; k < nout; ++k) total[k] = total[k] + part[k];
+    }
+}
+
+typedef struct {
+    const double *x_old, *av;
+    double x_cm[3], v_cm[3], m;
+} zone_ctx;
+
+static void term_pos_vel(const void *c, int32_t p, double *out, int nout) {
+    const zone_ctx *z = (const zone_ctx *)c;
+    (void)nout;
+    for (int i = 0; i < 3; ++i) { out[i] = z->x_old[3 * p + i]; out[3 + i] = z->av[3 * p + i]; }
+}
+static void term_ang_mom(const void *c, int32_t p, double *out, int nout) {
+    const zone_ctx *z = (const zone_ctx *)c;
+    double dx[3], dv[3], Li[3];
+    (void)nout;
+    sub3(&z->x_old[3 * p], z->x_cm, dx);
+    sub3(&z->av[3 * p], z->v_cm, dv);
+    cross3(dx, dv, Li);
+    for (int i = 0; i < 3; ++i) out[i] = z->m * Li[i];
+}
+static void term_inertia(const void *c, int32_t p, double *out, int nout) {
+    const zone_ctx *z = (const zone_ctx *)c;
+    double dx[3], mag_dx, tmp;
+    (void)nout;
+    sub3(&z->x_old[3 * p], z->x_cm, dx);
+    mag_dx = mag3(dx);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            tmp = -dx[i] * dx[j];
+            if (i == j) tmp += mag_dx * mag_dx;
+            out[3 * i + j] = tmp * z->m;
+        }
+}
+
+/* updateImpactListVelocity (dcollid3d.cpp:70-200) on members[0..n); the
+ * sums x_cm, v_cm, L = sum m dx x dv, I = sum m (|dx|^2 delta - dx dx^T) are
+ * accumulated in the order selected by `lanes` (see ordered_sum) */
 static int zone_rigid(const orc_params *P, int32_t n, const int32_t *mem, const double *x_old,
-                      double *av, const uint8_t *fixed) {
+                      double *av, const uint8_t *fixed, int lanes) {
     double m = P->mass, dt = P->dt;
     int mode = P->math_mode;
-    double x_cm[3] = {0, 0, 0}, v_cm[3] = {0, 0, 0}, L[3] = {0, 0, 0};
-    double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, tmp[3][3];
-    int32_t num_pts = 0;
-    for (int32_t q = 0; q < n; ++q) {
-        int32_t p = mem[q];
-        num_pts++;
-        for (int i = 0; i < 3; ++i) { x_cm[i] += x_old[3 * p + i]; v_cm[i] += av[3 * p + i]; }
-    }
-    for (int i = 0; i < 3; ++i) { x_cm[i] /= num_pts; v_cm[i] /= num_pts; }
-    for (int32_t q = 0; q < n; ++q) {
-        int32_t p = mem[q];
-        double dx[3], dv[3], Li[3];
-        sub3(&x_old[3 * p], x_cm, dx);
-        sub3(&av[3 * p], v_cm, dv);
-        cross3(dx, dv, Li);
-        for (int i = 0; i < 3; ++i) Li[i] = m * Li[i];
-        for (int i = 0; i < 3; ++i) L[i] = Li[i] + L[i];
-    }
-    for (int32_t q = 0; q < n; ++q) {
-        int32_t p = mem[q];
-        double dx[3], mag_dx;
-        sub3(&x_old[3 * p], x_cm, dx);
-        mag_dx = mag3(dx);
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                tmp[i][j] = -dx[i] * dx[j];
-                if (i == j) tmp[i][j] += mag_dx * mag_dx;
-                I[i][j] += tmp[i][j] * m;
-            }
-    }
+    zone_ctx z;
+    z.x_old = x_old; z.av = av; z.m = m;
+    double sums[9], L[3], I[3][3], tmp[3][3];
+    int32_t num_pts = n;
+    ordered_sum(lanes, n, mem, term_pos_vel, &z, 6, sums);
+    for (int i = 0; i < 3; ++i) { z.x_cm[i] = sums[i] / num_pts; z.v_cm[i] = sums[3 + i] / num_pts; }
+    ordered_sum(lanes, n, mem, term_ang_mom, &z, 3, sums);
+    for (int i = 0; i < 3; ++i) L[i] = sums[i];
+    ordered_sum(lanes, n, mem, term_inertia, &z, 9, sums);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) I[i][j] = sums[3 * i + j];
+    const double *x_cm = z.x_cm, *v_cm = z.v_cm;
     double w[3], mag_w;
     for (int i = 0; i < 3; ++i) {
         memcpy(tmp, I, sizeof(tmp));
@@ -1075,13 +1121,13 @@ static int32_t update_impact_zone_velocity(orc_solver *s) {
                 if (s->sorted[pt] || s->uf_w[uf_find(s, pt)] == 1) continue;
                 int32_t head = uf_find(s, pt), n = 0;
                 for (int32_t q = head; q >= 0; q = s->uf_next[q]) { mem[n++] = q; s->sorted[q] = 1; }
-                if (zone_rigid(&s->P, n, mem, s->x_old, s->av, s->fixed)) s->err = ORC_ERR_NONFINITE;
+                if (zone_rigid(&s->P, n, mem, s->x_old, s->av, s->fixed, 0)) s->err = ORC_ERR_NONFINITE;
                 nz++;
             }
         }
     } else {
         /* canonical: zones in order of their smallest vertex, members in
-         * ascending vertex id */
+         * ascending vertex id, sums in the 64-lane order of ordered_sum */
         int32_t *root_of = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->nv);
         int32_t *cnt = (int32_t *)calloc((size_t)s->nv + 1, sizeof(int32_t));
         int32_t *start = (int32_t *)malloc(sizeof(int32_t) * ((size_t)s->nv + 1));
@@ -1096,7 +1142,7 @@ static int32_t update_impact_zone_velocity(orc_solver *s) {
             int32_t r = root_of[v];
             if (done[r] || cnt[r] < 2) continue;
             done[r] = 1;
-            if (zone_rigid(&s->P, cnt[r], &mem[start[r]], s->x_old, s->av, s->fixed))
+            if (zone_rigid(&s->P, cnt[r], &mem[start[r]], s->x_old, s->av, s->fixed, 64))
                 s->err = ORC_ERR_NONFINITE;
             nz++;
         }
@@ -1314,10 +1360,10 @@ int orc_edge_to_edge(const orc_params *p, const double *c, const double *x_old,
     return unit_subtest(p, c, x_old, avg_vel, is_fixed, h, imp, fric, counts, 0);
 }
 int orc_zone_rigid_update(const orc_params *p, int32_t n, const double *x_old, double *avg_vel,
-                          const uint8_t *is_fixed) {
+                          const uint8_t *is_fixed, int32_t lanes) {
     int32_t *mem = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
     for (int32_t i = 0; i < n; ++i) mem[i] = i;
-    int err = zone_rigid(p, n, mem, x_old, avg_vel, is_fixed);
+    int err = zone_rigid(p, n, mem, x_old, avg_vel, is_fixed, lanes);
     free(mem);
     return err ? ORC_ERR_NONFINITE : ORC_OK;
 }

@@ -122,9 +122,10 @@ int orc_point_to_tri(const orc_params *p, const double *c, const double *x_old,
 int orc_edge_to_edge(const orc_params *p, const double *c, const double *x_old,
                      const double *avg_vel, const uint8_t *is_fixed, double h,
                      double *imp, double *fric, int32_t *counts);
-/* updateImpactListVelocity on one zone given in member order */
+/* updateImpactListVelocity on one zone given in member order; lanes = 0
+ * sums members sequentially (reference), 64 in the canonical lane order */
 int orc_zone_rigid_update(const orc_params *p, int32_t n, const double *x_old,
-                          double *avg_vel, const uint8_t *is_fixed);
+                          double *avg_vel, const uint8_t *is_fixed, int32_t lanes);
 /* elementary functions of the deterministic math mode */
 double orc_math_acos(double x);
 double orc_math_cos(double x);

@@ -88,7 +88,7 @@ def lib():
         L.orc_is_coplanar.argtypes = [P(d), P(d), d, i32, P(d)]
         for fn in (L.orc_point_to_tri, L.orc_edge_to_edge):
             fn.argtypes = [P(Params), P(d), P(d), P(d), P(u8), d, P(d), P(d), P(i32)]
-        L.orc_zone_rigid_update.argtypes = [P(Params), i32, P(d), P(d), P(u8)]
+        L.orc_zone_rigid_update.argtypes = [P(Params), i32, P(d), P(d), P(u8), i32]
         for fn in (L.orc_math_acos, L.orc_math_cos, L.orc_math_sin, L.orc_math_cbrt):
             fn.argtypes = [d]
             fn.restype = d
@@ -236,11 +236,11 @@ def edge_to_edge(params, c, x_old, avg_vel, fixed=None, h=None):
     return _unit(lib().orc_edge_to_edge, params, c, x_old, avg_vel, fixed, h)
 
 
-def zone_rigid_update(params, x_old, avg_vel, fixed=None):
+def zone_rigid_update(params, x_old, avg_vel, fixed=None, lanes=0):
     xo = np.ascontiguousarray(x_old, np.float64)
     av = np.array(avg_vel, np.float64, copy=True, order="C")
     n = xo.shape[0]
     fx = np.ascontiguousarray(np.zeros(n) if fixed is None else fixed, np.uint8)
     rc = lib().orc_zone_rigid_update(C.byref(params), n, _ptr(xo, C.c_double),
-                                     _ptr(av, C.c_double), _ptr(fx, C.c_uint8))
+                                     _ptr(av, C.c_double), _ptr(fx, C.c_uint8), lanes)
     return rc, av

@@ -25,8 +25,9 @@ def sorted_log(log):
     return log[order]
 
 
-def run_parity(scene, steps, check_every=1, compare_zones=True, verbose=False, **kw):
-    gpu = solver_for_scene(scene, log_pairs=1, **kw)
+def run_parity(scene, steps, check_every=1, compare_zones=True, verbose=False, gpu_kw=None,
+               **kw):
+    gpu = solver_for_scene(scene, log_pairs=1, **(gpu_kw or {}), **kw)
     orc = O.OracleSolver(scene, log_pairs=1, order_mode=O.ORDER_CANONICAL,
                          math_mode=O.MATH_DET, **kw)
     stats = dict(steps=0, hit_pairs=0, zone_steps=0, strain_sweeps=0)

@@ -32,11 +32,14 @@ def test_config1_reduced_self_collision():
     assert st["hit_pairs"] > 0
 
 
-def test_config4_reduced_cloth_strings_boxes():
-    """Config 4 geometry at 96x96 cloth, 60 strings, friction 0.3."""
+@pytest.mark.parametrize("full", [0, 1], ids=["incremental", "full_passes"])
+def test_config4_reduced_cloth_strings_boxes(full):
+    """Config 4 geometry at 96x96 cloth, 60 strings, friction 0.3; with the
+    incremental later passes (default) and with every pass recomputed."""
     sc = S.combined_scene(n=96, nstrings=60, steps=60)
-    st = run_parity(sc, steps=60, check_every=5, verbose=True)
+    st = run_parity(sc, steps=60, check_every=5, verbose=True, gpu_kw=dict(force_full_passes=full))
     assert st["hit_pairs"] > 0
+    assert st["zone_steps"] > 0
 
 
 @pytest.mark.parametrize("maker", [S.deck_ball_ball, S.deck_string_box, S.deck_string_string,
