@@ -50,7 +50,8 @@ class MeshView(C.Structure):
 class PassInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("index", C.c_int32), ("candidates", C.c_int64),
                 ("narrow_pairs", C.c_int64), ("hit_pairs", C.c_int64),
-                ("hit_subtests", C.c_int64), ("ms_broad", C.c_double),
+                ("hit_subtests", C.c_int64), ("tested_pairs", C.c_int64),
+                ("incremental", C.c_int32), ("pad", C.c_int32), ("ms_broad", C.c_double),
                 ("ms_narrow", C.c_double), ("ms_response", C.c_double),
                 ("ms_traverse_kernel", C.c_double), ("ms_narrow_kernel", C.c_double)]
 
@@ -60,10 +61,11 @@ class StepReport(C.Structure):
                 ("n_zone_passes", C.c_int32), ("has_collision", C.c_int32),
                 ("n_zones", C.c_int32), ("strain_sweeps", C.c_int32),
                 ("strain_edges", C.c_int32), ("degenerate_edges", C.c_int32),
-                ("pad", C.c_int32), ("candidates", C.c_int64), ("narrow_pairs", C.c_int64),
+                ("strain_levels", C.c_int32), ("candidates", C.c_int64), ("narrow_pairs", C.c_int64),
                 ("hit_pairs", C.c_int64), ("hit_subtests", C.c_int64),
                 ("ms_total", C.c_double), ("ms_broad", C.c_double), ("ms_narrow", C.c_double),
-                ("ms_response", C.c_double), ("ms_zone", C.c_double), ("ms_strain", C.c_double)]
+                ("ms_response", C.c_double), ("ms_zone", C.c_double), ("ms_strain", C.c_double),
+                ("strain_slots", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}

@@ -94,9 +94,9 @@ void free_dev(DevState& S) {
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
   dfree(S.zkey); dfree(S.zkey_tmp); dfree(S.zval); dfree(S.zval_tmp); dfree(S.zsize);
   dfree(S.zhead); dfree(S.zend);
-  dfree(S.lvl_ptr); dfree(S.lvl_elems); dfree(S.plog); dfree(S.counters);
-  dfree(S.elem_level); dfree(S.next_level); dfree(S.elem_active); dfree(S.lvl_active);
-  dfree(S.vdirty);
+  dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters);
+  dfree(S.shead); dfree(S.sbody); dfree(S.sflag);
+  dfree(S.lvl_active);
   S = DevState();
 }
 
@@ -227,6 +227,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   const cs_params& P = h->P;
   const int ccd = kind != CS_PASS_PROXIMITY;
   const bool incremental = ccd && h->prev.valid && !P.force_full_passes;
+  const bool exact_counts = P.log_pairs != 0;
   PassTimes T{};
   // the previous pass's results become "prev"; this pass fills the others
   swap_ptr(S.pairs, S.pairs_prev);
@@ -268,7 +269,10 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     launch_mark_dirty(h->st, S);
     HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
     mark(h, &T.t0);
-    launch_query_dirty(h->st, S, 0);                 // previous boxes: count only
+    // previous boxes: count only.  The count only feeds the exact candidate
+    // total of the pass (a diagnostic), so it runs when the caller asks for
+    // the parity taps; otherwise the candidate total is reported as -1.
+    if (exact_counts) launch_query_dirty(h->st, S, 0);
     launch_dirty_element_boxes(h->st, S, P.dt);
     HIPCHK(h, refit_bvh(h->st, S, true));
     launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
@@ -304,7 +308,8 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     if (nq > 0)
       HIPCHK(h, hipMemcpyAsync(S.pairs + carried_pairs, S.qpairs, sizeof(int2) * (size_t)nq,
                                hipMemcpyDeviceToDevice, h->st));
-    cand = prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64);
+    cand = exact_counts && prev.cand != ~0ull ? prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64)
+                                              : ~0ull;
   }
   mark(h, &T.e1);
   NarrowParams NP;
@@ -359,8 +364,10 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   cs_pass_info pi{};
   pi.kind = kind;
   pi.index = index;
-  pi.candidates = (int64_t)cand;
+  pi.candidates = cand == ~0ull ? -1 : (int64_t)cand;
   pi.narrow_pairs = npairs;
+  pi.tested_pairs = narrow_n;
+  pi.incremental = incremental ? 1 : 0;
   pi.hit_pairs = nhp;
   pi.hit_subtests = h->h_counters[CNT_HIT_SUB];
   pi.ms_broad = elapsed(T.e0, T.e1);
@@ -526,28 +533,56 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     nlev = std::max(nlev, l + 1);
     ++nsched;
   }
-  // level of each vertex's next visit (elements in order, rigid ones skipped)
-  std::vector<int4> nextl((size_t)std::max(ne, 1), make_int4(-1, -1, -1, -1));
-  {
-    std::vector<int> nxt((size_t)nv, -1);
-    for (int e = ne - 1; e >= 0; --e) {
-      if (level[e] < 0) continue;
-      const int np = (ef[e] & EF_TRI) ? 3 : 2;
-      const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
-      int nl[3] = {-1, -1, -1};
-      for (int i = 0; i < np; ++i) nl[i] = nxt[vv[i]];
-      nextl[e] = make_int4(nl[0], nl[1], nl[2], -1);
-      for (int i = 0; i < np; ++i) nxt[vv[i]] = level[e];
-    }
-  }
-  std::vector<int> lptr((size_t)nlev + 1, 0), lel((size_t)std::max(nsched, 1));
+  // slots: the scheduled elements in level order; within a level, element
+  // order (any order is exact there: the elements of one level share no
+  // vertex)
+  std::vector<int> lptr((size_t)nlev + 1, 0), slot_of((size_t)std::max(ne, 1), -1);
+  const size_t nslot = (size_t)std::max(nsched, 1);
   for (int e = 0; e < ne; ++e)
     if (level[e] >= 0) lptr[level[e] + 1]++;
   for (int l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
   {
     std::vector<int> fill(lptr.begin(), lptr.end() - 1);
     for (int e = 0; e < ne; ++e)
-      if (level[e] >= 0) lel[fill[level[e]]++] = e;
+      if (level[e] >= 0) slot_of[e] = fill[level[e]]++;
+  }
+  std::vector<StrainHead> shead(nslot);
+  std::vector<StrainBody> sbody(nslot);
+  std::memset(shead.data(), 0xff, sizeof(StrainHead) * nslot);
+  std::memset(sbody.data(), 0, sizeof(StrainBody) * nslot);
+  {
+    // the next visit of each vertex: the next scheduled element holding it
+    // (element order and level order agree on the elements of one vertex)
+    std::vector<int> nxt_slot((size_t)nv, -1), nxt_pos((size_t)nv, -1);
+    for (int e = ne - 1; e >= 0; --e) {
+      if (level[e] < 0) continue;
+      const bool tri = (ef[e] & EF_TRI) != 0;
+      const int np = tri ? 3 : 2;
+      const int vv[3] = {ev[e].x, ev[e].y, tri ? ev[e].z : -1};
+      const int s = slot_of[e];
+      StrainHead& hd = shead[s];
+      for (int i = 0; i < 3; ++i) {
+        hd.v[i] = vv[i];
+        hd.nl[i] = -1;
+        hd.ns[i] = -1;
+      }
+      for (int i = 0; i < np; ++i) {
+        const int ns = nxt_slot[vv[i]];
+        if (ns >= 0) {
+          hd.ns[i] = 4 * ns + nxt_pos[vv[i]];  // byte of the next slot's flag word
+          hd.nl[i] = shead[ns].level;
+        }
+      }
+      hd.level = level[e];
+      hd.elem = e;
+      hd.pad = 0;
+      for (int i = 0; i < np; ++i) {
+        nxt_slot[vv[i]] = s;
+        nxt_pos[vv[i]] = i;
+      }
+      StrainBody& bd = sbody[s];
+      bd.L0[0] = l0[e].x; bd.L0[1] = l0[e].y; bd.L0[2] = l0[e].z;
+    }
   }
 
   HIPCHK(h, hipSetDevice(h->P.device));
@@ -587,10 +622,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.zhead, nv)); HIPCHK(h, dalloc(&S.zend, nv));
   S.n_levels = nlev;
   S.n_sched = nsched;
-  HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1)); HIPCHK(h, dalloc(&S.lvl_elems, std::max(nsched, 1)));
-  HIPCHK(h, dalloc(&S.elem_level, ne)); HIPCHK(h, dalloc(&S.next_level, ne));
-  HIPCHK(h, dalloc(&S.elem_active, ne)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
-  HIPCHK(h, dalloc(&S.vdirty, nv));
+  HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1));
+  HIPCHK(h, dalloc(&S.shead, nslot)); HIPCHK(h, dalloc(&S.sbody, nslot));
+  HIPCHK(h, dalloc(&S.sflag, nslot)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
   S.plog_cap = 1 << 20;
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
   HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
@@ -600,9 +634,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.len0, l0.data(), sizeof(dvec4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.fixed, fixed.data(), nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.lvl_ptr, lptr.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
-  if (nsched) HIPCHK(h, hipMemcpy(S.lvl_elems, lel.data(), sizeof(int) * nsched, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(S.elem_level, level.data(), sizeof(int) * ne, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(S.next_level, nextl.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.shead, shead.data(), sizeof(StrainHead) * nslot, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.sbody, sbody.data(), sizeof(StrainBody) * nslot, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(S.sflag, 0, sizeof(uint32_t) * nslot));
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
   HIPCHK(h, hipMemset(S.has, 0, nv));
   HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
@@ -715,6 +749,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
     launch_domain_boundary(h->st, S, P.dom_lo, P.dom_hi, P.dt, P.friction, P.thickness);
     launch_final_position(h->st, S, P.dt);
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_FLAG, 0, sizeof(int) * 4, h->st));
+    HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_LEVELS, 0, sizeof(int) * 2, h->st));
     HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps, nullptr));
     launch_final_velocity(h->st, S);
     HIPCHK(h, hipGetLastError());
@@ -726,6 +761,8 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
     R.strain_sweeps = h->h_counters[CNT_STRAIN_SWEEPS];
     R.strain_edges = h->h_counters[CNT_STRAIN_EDGES];
     R.degenerate_edges = h->h_counters[CNT_STRAIN_DEGEN];
+    R.strain_levels = h->h_counters[CNT_STRAIN_LEVELS];
+    R.strain_slots = h->h_counters[CNT_STRAIN_EVALS];
     int e = device_error(h);
     if (e) status = e;
   }
@@ -733,7 +770,8 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   R.n_passes = (int)h->passes.size();
   R.has_collision = h->has_collision;
   for (auto& p : h->passes) {
-    R.candidates += p.candidates;
+    if (p.candidates >= 0 && R.candidates >= 0) R.candidates += p.candidates;
+    else R.candidates = -1;
     R.narrow_pairs += p.narrow_pairs;
     R.hit_pairs += p.hit_pairs;
     R.hit_subtests += p.hit_subtests;

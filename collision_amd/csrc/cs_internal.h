@@ -38,6 +38,8 @@ enum {
   CNT_STRAIN_DEGEN = 10,
   CNT_STRAIN_SWEEPS = 11,
   CNT_DIRTY = 12,        // dirty elements
+  CNT_STRAIN_LEVELS = 13,  // levels evaluated by the strain sweeps of the step
+  CNT_STRAIN_EVALS = 14,   // slots evaluated by the strain sweeps of the step
   CNT_ERR = 15,
   CNT_CAND64 = 16,       // box-overlap candidates of a full traversal
   CNT_OLD64 = 18,        // candidates touching dirty elements, previous boxes
@@ -54,6 +56,27 @@ struct NarrowParams {
 };
 
 // All device memory of one solver handle.
+// One slot of the strain-limit schedule (one non-rigid element).  The head
+// (48 B) is what every visit of the slot reads; the body (128 B) is read only
+// when the slot is evaluated.  Rest lengths are written at assembly; x_old of
+// the vertices and the x_old edge lengths are packed once per step.
+struct alignas(16) StrainHead {
+  int v[3];    // vertex ids; v[2] = -1 for a bond
+  int nl[3];   // level of each vertex's next visit (-1: none)
+  int ns[3];   // byte index (4 * slot + position) of the next visit's dirty flag (-1: none)
+  int level;
+  int elem;    // element id
+  int pad;
+};
+static_assert(sizeof(StrainHead) == 48, "StrainHead layout");
+struct alignas(16) StrainBody {
+  double x[9];     // x_old of vertices 0, 1, 2 (vertex 2 unused for a bond)
+  double lold[3];  // x_old lengths of edges (0,1), (1,2), (2,0)
+  double L0[3];    // rest lengths of the same edges
+  double unused;
+};
+static_assert(sizeof(StrainBody) == 128, "StrainBody layout");
+
 struct DevState {
   int nv = 0, nt = 0, nb = 0, ne = 0;
   // vertices
@@ -107,13 +130,13 @@ struct DevState {
   int32_t* zsize = nullptr;
   int32_t* zhead = nullptr;  // first sorted index of each zone
   int32_t* zend = nullptr;   // one past the last sorted index, by zone root
-  // strain-limit schedule (level-ordered element lists)
-  int32_t *lvl_ptr = nullptr, *lvl_elems = nullptr;
-  int32_t* elem_level = nullptr;   // level of each element (-1: rigid, not swept)
-  int4* next_level = nullptr;      // per element vertex: level of the vertex's next visit
-  uint8_t* elem_active = nullptr;  // element has an edge over the limit (pre-check)
+  // strain-limit schedule: slots in level order, slots of level L are
+  // [lvl_ptr[L], lvl_ptr[L+1])
+  int32_t* lvl_ptr = nullptr;
+  StrainHead* shead = nullptr;
+  StrainBody* sbody = nullptr;
+  uint32_t* sflag = nullptr;       // per slot: bytes 0-2 vertex dirty, byte 3 pre-check active
   uint8_t* lvl_active = nullptr;   // level holds an active element (pre-check)
-  uint8_t* vdirty = nullptr;       // vertex written earlier in the current sweep
   int n_levels = 0, n_sched = 0;
   // logs
   int4* plog = nullptr;

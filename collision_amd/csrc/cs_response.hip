@@ -439,13 +439,13 @@ void launch_uf_init(hipStream_t st, DevState& S) {
 
 // ---------------------------------------------------------------------------
 // strain limiting
-__device__ inline bool edge_over_limit(const dvec4& o0, const dvec4& o1, const dvec4& a0,
-                                       const dvec4& a1, double len0, double dt, bool* degenerate) {
-  const double xc0[3] = {o0.x + dt * a0.x, o0.y + dt * a0.y, o0.z + dt * a0.z};
-  const double xc1[3] = {o1.x + dt * a1.x, o1.y + dt * a1.y, o1.z + dt * a1.z};
-  const double p0[3] = {o0.x, o0.y, o0.z}, p1[3] = {o1.x, o1.y, o1.z};
+// edge test with the x_old length given (it is fixed during the step)
+__device__ inline bool edge_over_limit_l(const double* o0, const double* o1, const dvec4& a0,
+                                         const dvec4& a1, double len_old, double len0, double dt,
+                                         bool* degenerate) {
+  const double xc0[3] = {o0[0] + dt * a0.x, o0[1] + dt * a0.y, o0[2] + dt * a0.z};
+  const double xc1[3] = {o1[0] + dt * a1.x, o1[1] + dt * a1.y, o1[2] + dt * a1.z};
   const double len_new = dist3(xc0, xc1);
-  const double len_old = dist3(p0, p1);
   *degenerate = false;
   if (len_old > CS_ROUND_EPS && len_new > CS_ROUND_EPS) {
     const double strain_rate = (len_new - len_old) / len_old;
@@ -458,54 +458,147 @@ __device__ inline bool edge_over_limit(const dvec4& o0, const dvec4& o1, const d
 
 // Sparse exact sweep.  An element none of whose vertices was written earlier
 // in the sweep reads the pre-sweep velocities, so whether it writes is
-// decided by the parallel pre-check (elem_active).  Only pre-check-active
-// elements and later visits of written ("dirty") vertices are evaluated, in
-// level order; a per-level flag in LDS (set by the pre-check and, on every
-// dirty vertex, for the level of the vertex's next visit) lets the single
-// work-group skip idle levels without touching memory.
+// decided by the parallel pre-check.  Only pre-check-active elements and
+// later visits of written ("dirty") vertices are evaluated, in level order.
+//
+// Dirty state travels along each vertex's chain of visits: a slot that
+// evaluates a dirty vertex sets the flag byte of that vertex in the slot of
+// its next visit (and the LDS flag of that slot's level), so every later
+// element holding the vertex is evaluated, as in a sequential sweep.  A
+// slot's flag word (bytes 0-2 dirty vertices, byte 3 pre-check active) and
+// head are one coalesced read per level; only evaluated slots read their
+// body (x_old, lengths) and velocities.  Levels whose LDS flag is clear are
+// skipped without touching memory.
 
-// pre-check with the current velocities: elem_active / lvl_active / any flag
-__global__ void k_strain_check(const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
-                               const dvec4* __restrict__ len0, const dvec4* __restrict__ x_old,
-                               const dvec4* __restrict__ av, const int32_t* __restrict__ elem_level,
-                               double dt, int ne, uint8_t* __restrict__ elem_active,
+// x_old of the slot vertices and edge lengths, once per step
+__global__ void k_strain_pack(const StrainHead* __restrict__ head, StrainBody* __restrict__ body,
+                              const dvec4* __restrict__ x_old, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const StrainHead h = head[s];
+  StrainBody& r = body[s];
+  const dvec4 a = x_old[h.v[0]], b = x_old[h.v[1]];
+  const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+  r.x[0] = a.x; r.x[1] = a.y; r.x[2] = a.z;
+  r.x[3] = b.x; r.x[4] = b.y; r.x[5] = b.z;
+  r.lold[0] = dist3(pa, pb);
+  if (h.v[2] >= 0) {
+    const dvec4 c = x_old[h.v[2]];
+    const double pc[3] = {c.x, c.y, c.z};
+    r.x[6] = c.x; r.x[7] = c.y; r.x[8] = c.z;
+    r.lold[1] = dist3(pb, pc);
+    r.lold[2] = dist3(pc, pa);
+  }
+}
+
+// pre-check with the current velocities: flag words (dirty bytes cleared,
+// active byte set), lvl_active and the any-edge flag
+__global__ void k_strain_check(const StrainHead* __restrict__ head,
+                               const StrainBody* __restrict__ body, const dvec4* __restrict__ av,
+                               double dt, int n, uint32_t* __restrict__ sflag,
                                uint8_t* __restrict__ lvl_active, int* __restrict__ counters) {
   if (counters[CNT_STRAIN_FLAG] < 0) return;  // loop already finished
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   bool any = false;
-  if (e < ne && !(eflags[e] & EF_RIGID)) {
-    const int4 t = ev[e];
-    const int np = (t.z >= 0) ? 3 : 2;
-    const int vid[3] = {t.x, t.y, t.z};
-    const dvec4 l0 = len0[e];
-    const double L0[3] = {l0.x, l0.y, l0.z};
-    for (int j = 0; j < ((np == 2) ? 1 : np); ++j) {
-      const int p0 = vid[j % np], p1 = vid[(j + 1) % np];
-      bool deg;
-      if (edge_over_limit(x_old[p0], x_old[p1], av[p0], av[p1], L0[j], dt, &deg)) any = true;
+  if (s < n) {
+    const StrainHead h = head[s];
+    const StrainBody& r = body[s];
+    const dvec4 a0 = av[h.v[0]], a1 = av[h.v[1]];
+    bool deg;
+    if (edge_over_limit_l(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, &deg)) any = true;
+    if (h.v[2] >= 0) {
+      const dvec4 a2 = av[h.v[2]];
+      if (edge_over_limit_l(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, &deg)) any = true;
+      if (edge_over_limit_l(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, &deg)) any = true;
     }
-    elem_active[e] = any ? 1 : 0;
-    if (any) lvl_active[elem_level[e]] = 1;
+    sflag[s] = any ? 0x01000000u : 0u;
+    if (any) lvl_active[h.level] = 1;
   }
   if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(&counters[CNT_STRAIN_FLAG], 1);
 }
 
 constexpr int STRAIN_THREADS = 1024;
 constexpr int STRAIN_LDS_LEVELS = 65536;  // levels whose flags fit in LDS
+
+struct StrainCounts {
+  int* flag;
+  int* edges;
+  int* degen;
+};
+
+// one edge of the sweep: average the two velocities when over the limit
+__device__ inline bool strain_edge(const double* xa, const double* xb, dvec4& a, dvec4& b,
+                                   double lold, double L0, double dt, int ia, int ib, dvec4* av,
+                                   const StrainCounts& C) {
+  bool deg;
+  if (!edge_over_limit_l(xa, xb, a, b, lold, L0, dt, &deg)) return false;
+  double vt[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
+  vt[0] = 0.5 * vt[0];
+  vt[1] = 0.5 * vt[1];
+  vt[2] = 0.5 * vt[2];
+  const dvec4 nv4 = dvec4{vt[0], vt[1], vt[2], 0.0};
+  a = nv4;
+  b = nv4;
+  av[ia] = nv4;
+  av[ib] = nv4;
+  *C.flag = 1;
+  atomicAdd(C.edges, 1);
+  if (deg) atomicAdd(C.degen, 1);
+  return true;
+}
+
+// evaluate one slot's edges in order, (0,1), (1,2), (2,0), with the current
+// velocities (reduceSuperelast's per-element loop, dcollid.cpp:515-560);
+// returns whether the slot was evaluated
+__device__ inline bool strain_slot(const StrainHead& h, uint32_t f, const StrainBody* body, int s,
+                                   uint8_t* sflag_b, dvec4* av, uint8_t* act, bool use_lds,
+                                   double dt, const StrainCounts& C) {
+  bool d0 = (f & 0xffu) != 0, d1 = (f & 0xff00u) != 0, d2 = (f & 0xff0000u) != 0;
+  if (f == 0u) return false;  // not pre-check active and no dirty vertex
+  const bool tri = h.v[2] >= 0;
+  const StrainBody& r = body[s];
+  dvec4 a0 = av[h.v[0]], a1 = av[h.v[1]], a2{};
+  if (tri) a2 = av[h.v[2]];
+  if (strain_edge(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, h.v[0], h.v[1], av, C))
+    d0 = d1 = true;
+  if (tri) {
+    if (strain_edge(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, h.v[1], h.v[2], av, C))
+      d1 = d2 = true;
+    if (strain_edge(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, h.v[2], h.v[0], av, C))
+      d2 = d0 = true;
+  }
+  // every later visit of a dirty vertex must be evaluated
+  if (d0 && h.ns[0] >= 0) {
+    sflag_b[h.ns[0]] = 1;
+    if (use_lds) act[h.nl[0]] = 1;
+  }
+  if (d1 && h.ns[1] >= 0) {
+    sflag_b[h.ns[1]] = 1;
+    if (use_lds) act[h.nl[1]] = 1;
+  }
+  if (d2 && h.ns[2] >= 0) {
+    sflag_b[h.ns[2]] = 1;
+    if (use_lds) act[h.nl[2]] = 1;
+  }
+  return true;
+}
+
+// One work-group walks the levels in order; lane t owns slots b + t,
+// b + t + STRAIN_THREADS, ... of a level.
 __global__ void __launch_bounds__(STRAIN_THREADS)
-    k_strain_sweep(const int32_t* __restrict__ lvl_ptr, const int32_t* __restrict__ lvl_elems,
-                   int n_levels, const int4* __restrict__ ev, const dvec4* __restrict__ len0,
-                   const int4* __restrict__ next_level, const uint8_t* __restrict__ elem_active,
-                   const uint8_t* __restrict__ lvl_active_g, uint8_t* vdirty,
-                   const dvec4* __restrict__ x_old, dvec4* av, double dt,
-                   int* __restrict__ counters) {
+    k_strain_sweep(const int32_t* __restrict__ lvl_ptr, int n_levels,
+                   const StrainHead* __restrict__ head, const StrainBody* __restrict__ body,
+                   uint32_t* sflag, const uint8_t* __restrict__ lvl_active_g, dvec4* av,
+                   double dt, int* __restrict__ counters) {
   __shared__ uint8_t act[STRAIN_LDS_LEVELS];
-  __shared__ int s_flag, s_edges, s_degen, s_state;
+  __shared__ int s_flag, s_edges, s_degen, s_state, s_levels, s_evals;
   if (threadIdx.x == 0) {
     s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweep needed
     s_flag = 0;
     s_edges = 0;
     s_degen = 0;
+    s_levels = 0;
+    s_evals = 0;
   }
   __syncthreads();
   const int state = s_state;
@@ -518,73 +611,49 @@ __global__ void __launch_bounds__(STRAIN_THREADS)
     }
     return;
   }
+  const StrainCounts C{&s_flag, &s_edges, &s_degen};
   const bool use_lds = n_levels <= STRAIN_LDS_LEVELS;
   if (use_lds)
     for (int L = threadIdx.x; L < n_levels; L += STRAIN_THREADS) act[L] = lvl_active_g[L];
   __syncthreads();
+  uint8_t* sflag_b = reinterpret_cast<uint8_t*>(sflag);
+  int evals = 0;
   for (int L = 0; L < n_levels; ++L) {
     if (use_lds && !act[L]) continue;  // uniform: flags change only before a barrier
     const int b = lvl_ptr[L], e_end = lvl_ptr[L + 1];
-    for (int idx = b + threadIdx.x; idx < e_end; idx += STRAIN_THREADS) {
-      const int e = lvl_elems[idx];
-      const int4 t = ev[e];
-      const int np = (t.z >= 0) ? 3 : 2;
-      const int vid[3] = {t.x, t.y, t.z};
-      bool dirty[3] = {vdirty[t.x] != 0, vdirty[t.y] != 0, np == 3 && vdirty[t.z] != 0};
-      if (!elem_active[e] && !dirty[0] && !dirty[1] && !dirty[2]) continue;
-      const dvec4 l0 = len0[e];
-      const double L0[3] = {l0.x, l0.y, l0.z};
-      for (int j = 0; j < ((np == 2) ? 1 : np); ++j) {
-        const int s0 = j % np, s1 = (j + 1) % np;
-        const int p0 = vid[s0], p1 = vid[s1];
-        const dvec4 a0 = av[p0], a1 = av[p1];
-        bool deg;
-        if (edge_over_limit(x_old[p0], x_old[p1], a0, a1, L0[j], dt, &deg)) {
-          double vt[3] = {a0.x + a1.x, a0.y + a1.y, a0.z + a1.z};
-          vt[0] = 0.5 * vt[0];
-          vt[1] = 0.5 * vt[1];
-          vt[2] = 0.5 * vt[2];
-          const dvec4 nv4 = dvec4{vt[0], vt[1], vt[2], 0.0};
-          av[p0] = nv4;
-          av[p1] = nv4;
-          dirty[s0] = dirty[s1] = true;
-          s_flag = 1;
-          atomicAdd(&s_edges, 1);
-          if (deg) atomicAdd(&s_degen, 1);
-        }
-      }
-      // every later visit of a dirty vertex must be evaluated
-      const int4 nl = next_level[e];
-      const int nls[3] = {nl.x, nl.y, nl.z};
-      for (int q = 0; q < np; ++q) {
-        if (!dirty[q]) continue;
-        vdirty[vid[q]] = 1;
-        if (use_lds && nls[q] >= 0) act[nls[q]] = 1;
-      }
+    for (int s = b + threadIdx.x; s < e_end; s += STRAIN_THREADS) {
+      const uint32_t f = sflag[s];  // last written before an earlier barrier
+      const StrainHead h = head[s];
+      evals += strain_slot(h, f, body, s, sflag_b, av, act, use_lds, dt, C);
     }
+    if (threadIdx.x == 0) ++s_levels;
     __syncthreads();
   }
+  if (evals) atomicAdd(&s_evals, evals);
+  __syncthreads();
   if (threadIdx.x == 0) {
     counters[CNT_STRAIN_EDGES] = s_edges;
     counters[CNT_STRAIN_DEGEN] += s_degen;
     counters[CNT_STRAIN_SWEEPS] += 1;
+    counters[CNT_STRAIN_LEVELS] += s_levels;
+    counters[CNT_STRAIN_EVALS] += s_evals;
     counters[CNT_STRAIN_FLAG] = s_flag ? 0 : -1;  // next check decides; no edge -> finished
   }
 }
 
 hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, int* sweeps_out) {
   (void)sweeps_out;
-  const int ne = S.ne;
-  if (ne == 0 || S.n_sched == 0 || max_sweeps <= 0) return hipSuccess;
+  const int n = S.n_sched;
+  if (S.ne == 0 || n == 0 || max_sweeps <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_strain_pack, dim3((n + 255) / 256), dim3(256), 0, st, S.shead, S.sbody,
+                     S.x_old, n);
   for (int s = 0; s < max_sweeps; ++s) {
     CS_HIP_RET(hipMemsetAsync(S.lvl_active, 0, (size_t)S.n_levels, st));
-    CS_HIP_RET(hipMemsetAsync(S.vdirty, 0, (size_t)S.nv, st));
-    hipLaunchKernelGGL(k_strain_check, dim3((ne + 255) / 256), dim3(256), 0, st, S.ev, S.eflags,
-                       S.len0, S.x_old, S.av, S.elem_level, dt, ne, S.elem_active, S.lvl_active,
-                       S.counters);
+    hipLaunchKernelGGL(k_strain_check, dim3((n + 255) / 256), dim3(256), 0, st, S.shead, S.sbody,
+                       S.av, dt, n, S.sflag, S.lvl_active, S.counters);
     hipLaunchKernelGGL(k_strain_sweep, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
-                       S.lvl_elems, S.n_levels, S.ev, S.len0, S.next_level, S.elem_active,
-                       S.lvl_active, S.vdirty, S.x_old, S.av, dt, S.counters);
+                       S.n_levels, S.shead, S.sbody, S.sflag, S.lvl_active, S.av, dt,
+                       S.counters);
   }
   return hipGetLastError();
 }

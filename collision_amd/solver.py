@@ -171,7 +171,8 @@ class CollisionSolver3d:
         self._check(self.L.cs_pass_infos(self.h, arr, n))
         return [dict(kind=_lib.PASS_NAMES[p.kind], index=p.index, candidates=p.candidates,
                      narrow_pairs=p.narrow_pairs, hit_pairs=p.hit_pairs,
-                     hit_subtests=p.hit_subtests, ms_broad=p.ms_broad, ms_narrow=p.ms_narrow,
+                     hit_subtests=p.hit_subtests, tested_pairs=p.tested_pairs,
+                     incremental=p.incremental, ms_broad=p.ms_broad, ms_narrow=p.ms_narrow,
                      ms_response=p.ms_response, ms_traverse_kernel=p.ms_traverse_kernel,
                      ms_narrow_kernel=p.ms_narrow_kernel) for p in arr[:n]]
 

@@ -79,12 +79,19 @@ typedef struct cs_mesh_view {
 typedef struct cs_pass_info {
     int32_t kind;            /* cs_pass_kind */
     int32_t index;
-    int64_t candidates;      /* box-overlapping element pairs */
+    int64_t candidates;      /* box-overlapping element pairs; -1 in an incremental
+                                pass unless log_pairs is set (the exact count needs
+                                an extra query of the changed elements) */
     int64_t narrow_pairs;    /* pairs whose sub-tests ran */
     int64_t hit_pairs;       /* pairs whose narrow phase reported contact */
     int64_t hit_subtests;
+    int64_t tested_pairs;    /* pairs the narrow-phase kernel evaluated in this pass
+                                (all narrow_pairs in a full pass; only pairs touching
+                                changed vertices in an incremental pass) */
+    int32_t incremental;     /* 1 if the pass reused the previous pass's results */
+    int32_t pad;
     double ms_broad, ms_narrow, ms_response;  /* phase wall time on the handle's stream */
-    double ms_traverse_kernel;                /* LBVH traversal kernel alone */
+    double ms_traverse_kernel;                /* broad-phase kernels (traversal / re-query) */
     double ms_narrow_kernel;                  /* narrow-phase kernel alone */
 } cs_pass_info;
 
@@ -98,9 +105,11 @@ typedef struct cs_step_report {
     int32_t strain_sweeps;
     int32_t strain_edges;    /* edges over the limit in the last sweep */
     int32_t degenerate_edges;
-    int32_t pad;
-    int64_t candidates, narrow_pairs, hit_pairs, hit_subtests;
+    int32_t strain_levels;   /* schedule levels the strain sweeps evaluated (sum) */
+    int64_t candidates;      /* sum over passes; -1 when a pass reports -1 */
+    int64_t narrow_pairs, hit_pairs, hit_subtests;
     double ms_total, ms_broad, ms_narrow, ms_response, ms_zone, ms_strain;
+    int64_t strain_slots;    /* element visits the strain sweeps evaluated (sum) */
 } cs_step_report;
 
 typedef struct cs_solver cs_solver;

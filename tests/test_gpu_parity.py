@@ -48,3 +48,38 @@ def test_reference_deck_scenarios(maker):
     sc = maker()
     st = run_parity(sc, steps=min(sc.steps, 120), check_every=10)
     assert st["steps"] > 0
+
+
+def test_cpp_host_api_matches_oracle(tmp_path):
+    """The C++ CollisionSolver3d wrapper (include/collision_amd/CollisionSolver3d.hpp)
+    driving a cloth onto a static box gives the oracle's positions bit for bit."""
+    import os
+    import subprocess
+
+    from oracle import oracle as O
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "cloth_on_box"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "cloth_on_box.cpp"), "-o", str(exe),
+                    "-L", os.path.join(root, "collision_amd"), "-lcollision_amd",
+                    "-Wl,-rpath," + os.path.join(root, "collision_amd")], check=True)
+    out = tmp_path / "x.bin"
+    steps = 40
+    r = subprocess.run([str(exe), str(out), str(steps)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    x_gpu = np.fromfile(out, dtype=np.float64).reshape(-1, 3)
+    b = S._Builder()
+    x, t = S.cloth_grid(24, 0.5, origin=(-0.25, -0.25), z=0.012)
+    v = np.zeros_like(x)
+    v[:, 2] = -1.0
+    b.add_surface(x, v, t, fixed=False)
+    bx, bt = S.box_mesh((0.0, 0.0, -0.05), (0.2, 0.2, 0.1))
+    b.add_surface(bx, np.zeros_like(bx), bt, fixed=True)
+    sc = b.build("cloth_on_box", thickness=1e-3, dt=1e-3, steps=steps)
+    orc = O.OracleSolver(sc)
+    hits = 0
+    for _ in range(steps):
+        hits += orc.step().hit_pairs
+    assert hits > 0
+    assert np.array_equal(orc.get_state()["x"], x_gpu)
