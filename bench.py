@@ -164,6 +164,10 @@ def main():
         "candidates_per_full_pass": avg(full, "candidates"),
         "narrow_pairs_per_full_pass": avg(full, "narrow_pairs"),
         "tested_pairs_per_incremental_pass": avg(incr, "tested_pairs"),
+        "hit_subtests_per_incremental_pass": avg(incr, "hit_subtests"),
+        "strain_sweeps_per_step": sum(r["strain_sweeps"] for r in reports) / args.steps,
+        "strain_levels_per_step": sum(r["strain_levels"] for r in reports) / args.steps,
+        "strain_slots_per_step": sum(r["strain_slots"] for r in reports) / args.steps,
         "phase_ms_per_step": {
             k: sum(r[k] for r in reports) / args.steps
             for k in ("ms_total", "ms_broad", "ms_narrow", "ms_response", "ms_zone", "ms_strain")

@@ -85,7 +85,7 @@ void free_dev(DevState& S) {
   dfree(S.fixed); dfree(S.has); dfree(S.uf);
   dfree(S.ev); dfree(S.eflags); dfree(S.len0); dfree(S.ebox);
   dfree(S.bounds); dfree(S.mcode); dfree(S.mid); dfree(S.mcode_tmp); dfree(S.mid_tmp);
-  dfree(S.sort_hist); dfree(S.nodes); dfree(S.leaves); dfree(S.parent); dfree(S.lparent);
+  dfree(S.sort_hist); dfree(S.nodes); S.leaves = nullptr; dfree(S.parent); dfree(S.nright); dfree(S.split_right);
   dfree(S.rmin); dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
@@ -188,7 +188,6 @@ int device_error(cs_solver* h) {
   const int e = h->h_counters[CNT_ERR];
   if (e & CS_DEVERR_NONFINITE)
     return fail(h, CS_ERR_NONFINITE, "non-finite value in the collision step");
-  if (e & CS_DEVERR_STACK) return fail(h, CS_ERR_INTERNAL, "BVH traversal stack overflow");
   if (e) return fail(h, CS_ERR_INTERNAL, "device error flag set");
   return CS_OK;
 }
@@ -598,8 +597,11 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.bounds, 6));
   HIPCHK(h, dalloc(&S.mcode, ne)); HIPCHK(h, dalloc(&S.mid, ne));
   HIPCHK(h, dalloc(&S.mcode_tmp, ne)); HIPCHK(h, dalloc(&S.mid_tmp, ne));
-  HIPCHK(h, dalloc(&S.nodes, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.leaves, ne));
-  HIPCHK(h, dalloc(&S.parent, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.lparent, ne));
+  HIPCHK(h, dalloc(&S.nodes, 2 * ne - 1));
+  S.leaves = S.nodes + (ne - 1);
+  HIPCHK(h, dalloc(&S.parent, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.nright, std::max(ne - 1, 1)));
+  HIPCHK(h, dalloc(&S.split_right, std::max(ne - 1, 1)));
   HIPCHK(h, dalloc(&S.rmin, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.rmax, std::max(ne - 1, 1)));
   HIPCHK(h, dalloc(&S.arrive, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.top_start, std::max(ne - 1, 1)));
   S.pair_cap = (int)std::min<int64_t>(3ll * ne + 4096, INT32_MAX / 2);

@@ -9,11 +9,13 @@
 //     reference's min/max_{static,moving}_coord, dcollid.cpp:852-932).
 //   * k_bounds / k_morton : 30-bit Morton codes of FP32 box centres.
 //   * (radix sort of codes, cs_sort.hip)
-//   * k_karras        : radix tree over the sorted codes (index tie-break).
+//   * k_karras        : radix tree over the sorted codes (index tie-break),
+//     k_ropes         : escape links for stackless depth-first traversal.
 //   * k_refit_tiles / k_refit_top : bottom-up FP32 boxes rounded outward,
 //     so every FP64 overlap is also an FP32 overlap.
-//   * k_traverse      : one lane per leaf in Morton order walks the tree and
-//     only visits leaves after itself in that order (each pair once); a
+//   * k_traverse      : one lane per leaf in Morton order; it starts at its
+//     own leaf's rope, so it visits exactly the subtrees after itself in
+//     depth-first order (every pair once, no stack); a
 //     leaf overlap is confirmed with the exact FP64 boxes, then the
 //     reference's early rejects (shared vertex, same rigid surface;
 //     dcollid.cpp:762, 805; dcollid3d.cpp:209-214, 257-264, 279-284) run
@@ -140,10 +142,15 @@ __device__ inline int lcp_delta(const uint32_t* __restrict__ codes, int n, int i
   return __clz(a ^ b);
 }
 
-// Karras (2012) radix tree: internal nodes 0..n-2; child < 0 is leaf ~j.
+// Karras (2012) radix tree: internal nodes 0..n-2, leaf j is node n-1+j.
+// The node splitting after leaf g records its second child in
+// split_right[g]: that child is what depth-first order visits right after
+// any subtree whose last leaf is g (every split position 0..n-2 belongs to
+// exactly one internal node).
 __global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __restrict__ nodes,
-                         int32_t* __restrict__ parent, int32_t* __restrict__ lparent,
-                         int32_t* __restrict__ rmin, int32_t* __restrict__ rmax) {
+                         int32_t* __restrict__ parent, int32_t* __restrict__ nright,
+                         int32_t* __restrict__ split_right, int32_t* __restrict__ rmin,
+                         int32_t* __restrict__ rmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (lcp_delta(codes, n, i, i + 1) - lcp_delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -163,15 +170,26 @@ __global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __
   } while (t > 1);
   const int gamma = i + s * d + min(d, 0);
   const int lo = min(i, j), hi = max(i, j);
-  const int left = (lo == gamma) ? ~gamma : gamma;
-  const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  const int left = (lo == gamma) ? (n - 1) + gamma : gamma;
+  const int right = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
   nodes[i].left = left;
-  nodes[i].right = right;
+  nright[i] = right;
+  split_right[gamma] = right;
   rmin[i] = lo;
   rmax[i] = hi;
-  if (left < 0) lparent[~left] = i; else parent[left] = i;
-  if (right < 0) lparent[~right] = i; else parent[right] = i;
+  parent[left] = i;
+  parent[right] = i;
   if (i == 0) parent[0] = -1;
+}
+
+// rope of internal node i: the second child of the node splitting after the
+// subtree's last leaf (none after the last leaf)
+__global__ void k_ropes(bvh_node* __restrict__ nodes, const int32_t* __restrict__ rmax,
+                        const int32_t* __restrict__ split_right, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int last = rmax[i];
+  nodes[i].rope = (last == n - 1) ? -1 : split_right[last];
 }
 
 // Refit.  Every internal node covers a contiguous range [rmin, rmax] of
@@ -192,10 +210,10 @@ __device__ inline bool is_top(const int32_t* rmin, const int32_t* rmax, int node
   return (rmin[node] / REFIT_TILE) != (rmax[node] / REFIT_TILE);
 }
 
-__device__ inline void merge_children(bvh_node* nodes, const bvh_node* leaves, int node) {
-  const int l = nodes[node].left, r = nodes[node].right;
-  const bvh_node A = (l < 0) ? leaves[~l] : nodes[l];
-  const bvh_node B = (r < 0) ? leaves[~r] : nodes[r];
+__device__ inline void merge_children(bvh_node* nodes, const int32_t* __restrict__ nright,
+                                      int node) {
+  const bvh_node A = nodes[nodes[node].left];
+  const bvh_node B = nodes[nright[node]];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     nodes[node].lo[d] = fminf(A.lo[d], B.lo[d]);
@@ -209,10 +227,10 @@ __device__ inline uint32_t wg_arrive(uint32_t* p) {
 
 __global__ void __launch_bounds__(REFIT_THREADS)
     k_refit_tiles(const dbox* __restrict__ ebox, const uint32_t* __restrict__ sorted_ids, int n,
-                  bvh_node* leaves, bvh_node* nodes, const int32_t* __restrict__ parent,
-                  const int32_t* __restrict__ lparent, const int32_t* __restrict__ rmin,
-                  const int32_t* __restrict__ rmax, uint32_t* arrive,
-                  const uint8_t* __restrict__ tile_dirty) {
+                  bvh_node* nodes, const int32_t* __restrict__ parent,
+                  const int32_t* __restrict__ nright, const int32_t* __restrict__ split_right,
+                  const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
+                  uint32_t* arrive, const uint8_t* __restrict__ tile_dirty) {
   if (tile_dirty && !tile_dirty[blockIdx.x]) return;  // boxes of this tile are unchanged
   const int first = blockIdx.x * REFIT_TILE;
   const int last = min(n, first + REFIT_TILE);
@@ -226,30 +244,30 @@ __global__ void __launch_bounds__(REFIT_THREADS)
       lf.hi[d] = f32_up(b.hi[d]);
     }
     lf.left = e;
-    lf.right = j;
-    leaves[j] = lf;
+    lf.rope = (j == n - 1) ? -1 : split_right[j];
+    nodes[(n - 1) + j] = lf;
     if (n == 1) continue;
-    int node = lparent[j];
+    int node = parent[(n - 1) + j];
     while (node >= 0 && !is_top(rmin, rmax, node)) {
       if (wg_arrive(&arrive[node]) == 0u) break;  // first arrival: sibling not done
-      merge_children(nodes, leaves, node);
+      merge_children(nodes, nright, node);
       node = parent[node];
     }
   }
 }
 
 // top nodes with no top child: start points of the single-work-group refit
-__global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __restrict__ rmin,
-                           const int32_t* __restrict__ rmax, int n, int32_t* __restrict__ out,
-                           int* __restrict__ count) {
+__global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __restrict__ nright,
+                           const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
+                           int n, int32_t* __restrict__ out, int* __restrict__ count) {
   __shared__ int s_cnt, s_base;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool start = false;
   if (i < n - 1 && is_top(rmin, rmax, i)) {
-    const int l = nodes[i].left, r = nodes[i].right;
-    start = !(l >= 0 && is_top(rmin, rmax, l)) && !(r >= 0 && is_top(rmin, rmax, r));
+    const int l = nodes[i].left, r = nright[i];
+    start = !(l < n - 1 && is_top(rmin, rmax, l)) && !(r < n - 1 && is_top(rmin, rmax, r));
   }
   int slot = -1;
   if (start) slot = atomicAdd(&s_cnt, 1);
@@ -260,20 +278,21 @@ __global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __
 }
 
 __global__ void __launch_bounds__(1024)
-    k_refit_top(bvh_node* nodes, const bvh_node* leaves, const int32_t* __restrict__ parent,
-                const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
-                const int32_t* __restrict__ starts, int nstarts, uint32_t* arrive) {
+    k_refit_top(bvh_node* nodes, int n, const int32_t* __restrict__ parent,
+                const int32_t* __restrict__ nright, const int32_t* __restrict__ rmin,
+                const int32_t* __restrict__ rmax, const int32_t* __restrict__ starts, int nstarts,
+                uint32_t* arrive) {
   for (int k = threadIdx.x; k < nstarts; k += 1024) {
     int node = starts[k];
-    merge_children(nodes, leaves, node);
+    merge_children(nodes, nright, node);
     node = parent[node];
     while (node >= 0) {
       // arrivals needed = number of top children (the others are done)
-      const int l = nodes[node].left, r = nodes[node].right;
-      const uint32_t need = (uint32_t)((l >= 0 && is_top(rmin, rmax, l)) ? 1 : 0) +
-                            (uint32_t)((r >= 0 && is_top(rmin, rmax, r)) ? 1 : 0);
+      const int l = nodes[node].left, r = nright[node];
+      const uint32_t need = (uint32_t)((l < n - 1 && is_top(rmin, rmax, l)) ? 1 : 0) +
+                            (uint32_t)((r < n - 1 && is_top(rmin, rmax, r)) ? 1 : 0);
       if (wg_arrive(&arrive[node]) + 1u < need) break;
-      merge_children(nodes, leaves, node);
+      merge_children(nodes, nright, node);
       node = parent[node];
     }
   }
@@ -319,73 +338,58 @@ __device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
   return false;
 }
 
-constexpr int TRAV_STACK = 64;
 constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flush
 
 __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
-                                                           const bvh_node* __restrict__ leaves,
-                                                           const int32_t* __restrict__ rmax,
                                                            const dbox* __restrict__ ebox,
                                                            const int4* __restrict__ ev,
                                                            const uint8_t* __restrict__ eflags,
                                                            int n, int2* __restrict__ pairs, int cap,
                                                            int* __restrict__ pair_count,
-                                                           unsigned long long* __restrict__ cand_count,
-                                                           int* __restrict__ err) {
+                                                           unsigned long long* __restrict__ cand_count) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
   __shared__ int s_off[TRAV_THREADS];
   __shared__ unsigned long long s_cand[TRAV_THREADS / 64];
   __shared__ int s_base;
   const int t = threadIdx.x;
   const int i = blockIdx.x * TRAV_THREADS + t;
+  const int leaf0 = n - 1;  // index of leaf 0
   unsigned long long cand = 0;
   int nbuf = 0;
   if (i < n && n > 1) {
-    const bvh_node me = leaves[i];
+    const bvh_node me = nodes[leaf0 + i];
     const int e = me.left;
     const dbox mb = ebox[e];
     const int4 mt = ev[e];
     const uint8_t mf = eflags[e];
-    int stack[TRAV_STACK];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-      const int nd = stack[--sp];
+    // depth-first order after leaf i: every leaf j > i, each subtree once
+    int nd = me.rope;
+    while (nd >= 0) {
       const bvh_node node = nodes[nd];
-      const int ch[2] = {node.left, node.right};
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int cc = ch[c];
-        if (cc < 0) {
-          const int j = ~cc;
-          if (j <= i) continue;
-          const bvh_node lf = leaves[j];
-          if (!f32_overlap(lf, me.lo, me.hi)) continue;
-          const int e2 = lf.left;
-          const dbox ob = ebox[e2];
-          if (!f64_overlap(ob, mb)) continue;
-          ++cand;
-          const int4 ot = ev[e2];
-          const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
-          if (rej) continue;
-          const int2 pr = make_int2(min(e, e2), max(e, e2));
-          if (nbuf < TRAV_BUF) {
-            buf[nbuf * TRAV_THREADS + t] = pr;
-            ++nbuf;
-          } else {
-            const int slot = atomicAdd(pair_count, 1);
-            if (slot < cap) pairs[slot] = pr;
-          }
-        } else {
-          if (rmax[cc] <= i) continue;
-          if (!f32_overlap(nodes[cc], me.lo, me.hi)) continue;
-          if (sp < TRAV_STACK) {
-            stack[sp++] = cc;
-          } else {
-            atomicOr(err, CS_DEVERR_STACK);
-          }
-        }
+      if (!f32_overlap(node, me.lo, me.hi)) {
+        nd = node.rope;
+        continue;
+      }
+      if (nd < leaf0) {
+        nd = node.left;
+        continue;
+      }
+      nd = node.rope;
+      const int e2 = node.left;
+      const dbox ob = ebox[e2];
+      if (!f64_overlap(ob, mb)) continue;
+      ++cand;
+      const int4 ot = ev[e2];
+      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
+      if (rej) continue;
+      const int2 pr = make_int2(min(e, e2), max(e, e2));
+      if (nbuf < TRAV_BUF) {
+        buf[nbuf * TRAV_THREADS + t] = pr;
+        ++nbuf;
+      } else {
+        const int slot = atomicAdd(pair_count, 1);
+        if (slot < cap) pairs[slot] = pr;
       }
     }
   }
@@ -490,8 +494,8 @@ __global__ void k_dirty_boxes(const int4* __restrict__ ev, const dvec4* __restri
 // only (previous boxes, into CNT_OLD64); emit = 1: count into CNT_NEW64 and
 // append non-rejected pairs to qpairs.
 __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
-    const bvh_node* __restrict__ nodes, const bvh_node* __restrict__ leaves,
-    const dbox* __restrict__ ebox, const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
+    const bvh_node* __restrict__ nodes, const dbox* __restrict__ ebox,
+    const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
     const uint8_t* __restrict__ dirty_e, const int32_t* __restrict__ dlist,
     const int32_t* __restrict__ leaf_of, int n, int emit, int2* __restrict__ qpairs, int qcap,
     int* __restrict__ counters) {
@@ -500,6 +504,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
   __shared__ unsigned long long s_w[TRAV_THREADS / 64];
   __shared__ int s_off[TRAV_THREADS];
   const int t = threadIdx.x;
+  const int leaf0 = n - 1;
   const int ndirty = counters[CNT_DIRTY];
   const int rounds = (ndirty + (int)(gridDim.x * TRAV_THREADS) - 1) / (int)(gridDim.x * TRAV_THREADS);
   for (int round = 0; round < rounds; ++round) {
@@ -509,47 +514,40 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
   if (k < ndirty && n > 1) {
     const int e = dlist[k];
     const int si = leaf_of[e];
-    const bvh_node me = leaves[si];
+    const bvh_node me = nodes[leaf0 + si];
     const dbox mb = ebox[e];
     const int4 mt = ev[e];
     const uint8_t mf = eflags[e];
-    int stack[TRAV_STACK];
-    int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-      const int nd = stack[--sp];
+    int nd = 0;  // root
+    while (nd >= 0) {
       const bvh_node node = nodes[nd];
-      const int ch[2] = {node.left, node.right};
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const int cc = ch[c];
-        if (cc < 0) {
-          const int j = ~cc;
-          if (j == si) continue;
-          const bvh_node lf = leaves[j];
-          if (!f32_overlap(lf, me.lo, me.hi)) continue;
-          const int e2 = lf.left;
-          if (dirty_e[e2] && j < si) continue;  // counted from e2
-          const dbox ob = ebox[e2];
-          if (!f64_overlap(ob, mb)) continue;
-          ++cand;
-          if (!emit) continue;
-          const int4 ot = ev[e2];
-          const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
-          if (rej) continue;
-          const int2 pr = make_int2(min(e, e2), max(e, e2));
-          if (nbuf < TRAV_BUF) {
-            buf[nbuf * TRAV_THREADS + t] = pr;
-            ++nbuf;
-          } else {
-            const int slot = atomicAdd(&counters[CNT_QPAIRS], 1);
-            if (slot < qcap) qpairs[slot] = pr;
-          }
-        } else {
-          if (!f32_overlap(nodes[cc], me.lo, me.hi)) continue;
-          if (sp < TRAV_STACK) stack[sp++] = cc;
-          else atomicOr(&counters[CNT_ERR], CS_DEVERR_STACK);
-        }
+      if (!f32_overlap(node, me.lo, me.hi)) {
+        nd = node.rope;
+        continue;
+      }
+      if (nd < leaf0) {
+        nd = node.left;
+        continue;
+      }
+      const int j = nd - leaf0;
+      nd = node.rope;
+      if (j == si) continue;
+      const int e2 = node.left;
+      if (dirty_e[e2] && j < si) continue;  // counted from e2
+      const dbox ob = ebox[e2];
+      if (!f64_overlap(ob, mb)) continue;
+      ++cand;
+      if (!emit) continue;
+      const int4 ot = ev[e2];
+      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
+      if (rej) continue;
+      const int2 pr = make_int2(min(e, e2), max(e, e2));
+      if (nbuf < TRAV_BUF) {
+        buf[nbuf * TRAV_THREADS + t] = pr;
+        ++nbuf;
+      } else {
+        const int slot = atomicAdd(&counters[CNT_QPAIRS], 1);
+        if (slot < qcap) qpairs[slot] = pr;
       }
     }
   }
@@ -669,7 +667,7 @@ void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
 
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
   hipLaunchKernelGGL(k_query_dirty, dim3(std::min((S.ne + TRAV_THREADS - 1) / TRAV_THREADS, 256)),
-                     dim3(TRAV_THREADS), 0, st, S.nodes, S.leaves, S.ebox, S.ev, S.eflags,
+                     dim3(TRAV_THREADS), 0, st, S.nodes, S.ebox, S.ev, S.eflags,
                      S.dirty_e, S.dlist, S.leaf_of, S.ne, emit, S.qpairs, S.qpair_cap, S.counters);
 }
 
@@ -711,11 +709,15 @@ hipError_t build_bvh(hipStream_t st, DevState& S) {
   hipLaunchKernelGGL(k_leaf_of, dim3((n + 255) / 256), dim3(256), 0, st, S.sorted_ids, n, S.leaf_of);
   S.n_top_start = 0;
   if (n > 1) {
+    // every split position is written by k_karras; -1 (end) is a safe default
+    CS_HIP_RET(hipMemsetAsync(S.split_right, 0xff, sizeof(int32_t) * (size_t)(n - 1), st));
     hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.sorted_codes, n,
-                       S.nodes, S.parent, S.lparent, S.rmin, S.rmax);
+                       S.nodes, S.parent, S.nright, S.split_right, S.rmin, S.rmax);
+    hipLaunchKernelGGL(k_ropes, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.rmax,
+                       S.split_right, n);
     CS_HIP_RET(hipMemsetAsync(S.counters + CNT_TMP, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_mark_top, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.rmin,
-                       S.rmax, n, S.top_start, S.counters + CNT_TMP);
+    hipLaunchKernelGGL(k_mark_top, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.nright,
+                       S.rmin, S.rmax, n, S.top_start, S.counters + CNT_TMP);
     int cnt = 0;
     CS_HIP_RET(hipMemcpyAsync(&cnt, S.counters + CNT_TMP, sizeof(int), hipMemcpyDeviceToHost, st));
     CS_HIP_RET(hipStreamSynchronize(st));
@@ -729,11 +731,11 @@ hipError_t refit_bvh(hipStream_t st, DevState& S, bool only_dirty_tiles) {
   if (n == 0) return hipSuccess;
   if (n > 1) CS_HIP_RET(hipMemsetAsync(S.arrive, 0, sizeof(uint32_t) * (size_t)(n - 1), st));
   hipLaunchKernelGGL(k_refit_tiles, dim3((n + REFIT_TILE - 1) / REFIT_TILE), dim3(REFIT_THREADS), 0,
-                     st, S.ebox, S.sorted_ids, n, S.leaves, S.nodes, S.parent, S.lparent, S.rmin,
-                     S.rmax, S.arrive, only_dirty_tiles ? S.tile_dirty : nullptr);
+                     st, S.ebox, S.sorted_ids, n, S.nodes, S.parent, S.nright, S.split_right,
+                     S.rmin, S.rmax, S.arrive, only_dirty_tiles ? S.tile_dirty : nullptr);
   if (n > 1 && S.n_top_start > 0)
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, S.leaves, S.parent, S.rmin,
-                       S.rmax, S.top_start, S.n_top_start, S.arrive);
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, n, S.parent, S.nright,
+                       S.rmin, S.rmax, S.top_start, S.n_top_start, S.arrive);
   return hipGetLastError();
 }
 
@@ -741,9 +743,8 @@ void launch_traverse(hipStream_t st, DevState& S, int cap) {
   const int n = S.ne;
   if (n == 0) return;
   hipLaunchKernelGGL(k_traverse, dim3((n + TRAV_THREADS - 1) / TRAV_THREADS), dim3(TRAV_THREADS), 0,
-                     st, S.nodes, S.leaves,
-                     S.rmax, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
-                     (unsigned long long*)(S.counters + CNT_CAND64), S.counters + CNT_ERR);
+                     st, S.nodes, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
+                     (unsigned long long*)(S.counters + CNT_CAND64));
 }
 
 }  // namespace cs

@@ -25,13 +25,20 @@ struct __align__(16) dbox {  // FP64 element box, inflated by 1e-6
   double hi[3];
 };
 
-// LBVH node: FP32 box rounded outward + children (leaf ids are encoded as
-// ~sorted_leaf_index, i.e. negative).
+// LBVH node, one 32-byte record.  Internal nodes 0..n-2 and leaves
+// n-1..2n-2 (leaf j = Morton-sorted element j) share one array.
+//   internal: FP32 box of the subtree (rounded outward), left = first child,
+//             rope = next node after the subtree in depth-first order
+//   leaf:     FP32 box of the element (rounded outward), left = element id,
+//             rope as above
+// rope = -1 ends a traversal.  Depth-first traversal needs no stack: on an
+// overlap go to `left` (internal) or handle the leaf and go to `rope`; on a
+// miss go to `rope`.
 struct __align__(16) bvh_node {
   float lo[3];
   int32_t left;
   float hi[3];
-  int32_t right;
+  int32_t rope;
 };
 
 // A narrow-phase hit: one sub-test of one element pair that reported

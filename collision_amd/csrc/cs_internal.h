@@ -19,7 +19,6 @@ enum : uint8_t { EF_TRI = 1, EF_RIGID = 2 };
 // device error bits (OR-ed into DevState::err)
 enum : int {
   CS_DEVERR_NONFINITE = 1,   // where the reference calls clean_up(ERROR)
-  CS_DEVERR_STACK = 2,       // BVH traversal stack overflow
   CS_DEVERR_STRAIN = 4,      // strain-limit schedule inconsistency
 };
 
@@ -95,7 +94,11 @@ struct DevState {
   uint32_t* sort_hist = nullptr;
   size_t sort_hist_words = 0;
   bvh_node *nodes = nullptr, *leaves = nullptr;
-  int32_t *parent = nullptr, *lparent = nullptr, *rmin = nullptr, *rmax = nullptr;
+  // nodes: 2n-1 records (internal 0..n-2, then leaves); leaves = nodes + n-1
+  int32_t* parent = nullptr;       // parent of every node (internal and leaf), -1 at the root
+  int32_t* nright = nullptr;       // second child of each internal node
+  int32_t* split_right = nullptr;  // [g]: second child of the node splitting after leaf g
+  int32_t *rmin = nullptr, *rmax = nullptr;
   uint32_t* arrive = nullptr;
   int32_t* top_start = nullptr;  // nodes spanning several refit tiles whose children do not
   int n_top_start = 0;           // (built once per LBVH build)
