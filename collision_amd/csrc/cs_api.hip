@@ -89,10 +89,11 @@ void free_dev(DevState& S) {
   dfree(S.rmin); dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
-  dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist); dfree(S.tile_dirty);
+  dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist);
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
-  dfree(S.zkey); dfree(S.zkey_tmp); dfree(S.zval); dfree(S.zval_tmp); dfree(S.zsize);
+  dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
+  dfree(S.zkey); dfree(S.zval); dfree(S.zval_tmp);
   dfree(S.zhead); dfree(S.zend);
   dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters);
   dfree(S.shead); dfree(S.sbody); dfree(S.sflag);
@@ -273,7 +274,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // the parity taps; otherwise the candidate total is reported as -1.
     if (exact_counts) launch_query_dirty(h->st, S, 0);
     launch_dirty_element_boxes(h->st, S, P.dt);
-    HIPCHK(h, refit_bvh(h->st, S, true));
+    launch_grow_dirty(h->st, S);                     // leaves exact, ancestors grown
     launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
     mark(h, &T.t1);
     launch_carry(h->st, S, prev.npairs, prev.nhits, prev.nhp);
@@ -318,6 +319,10 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   NP.zone_mode = (kind == CS_PASS_ZONE);
   NP.log_pairs = P.log_pairs;
   NP.pass_id = pass_id;
+  // impact-zone merges of the carried hit pairs (union order does not
+  // change the partition; run first so the member count read after the
+  // narrow phase is complete)
+  if (incremental && kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, carried_hp, 1);
   const int2* narrow_in = incremental ? S.qpairs : S.pairs;
   const int narrow_n = incremental ? nq : npairs;
   int nhits = 0, nhp = 0;
@@ -348,12 +353,11 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     if ((rc = set_counter(h, CNT_HIT_PAIRS, carried_hp))) return rc;
     if ((rc = set_counter(h, CNT_HIT_SUB, carried_sub))) return rc;
   }
-  if (incremental && kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, carried_hp, 1);
   mark(h, &T.e2);
   HIPCHK(h, apply_hits(h->st, S, nhits));
   if (kind == CS_PASS_ZONE) {
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), h->st));
-    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt));
+    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, h->h_counters[CNT_ZLIST]));
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark(h, &T.e3);
@@ -617,10 +621,10 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.dirty_v, nv));
   HIPCHK(h, dalloc(&S.dirty_e, ne));
   HIPCHK(h, dalloc(&S.dlist, ne));
-  S.n_tiles = refit_tile_count(ne);
-  HIPCHK(h, dalloc(&S.tile_dirty, S.n_tiles));
-  HIPCHK(h, dalloc(&S.zkey, nv)); HIPCHK(h, dalloc(&S.zkey_tmp, nv));
-  HIPCHK(h, dalloc(&S.zval, nv)); HIPCHK(h, dalloc(&S.zval_tmp, nv)); HIPCHK(h, dalloc(&S.zsize, nv));
+  HIPCHK(h, dalloc(&S.zmark, nv)); HIPCHK(h, dalloc(&S.zlist, nv));
+  HIPCHK(h, dalloc(&S.zkey64, nv)); HIPCHK(h, dalloc(&S.zkey64_tmp, nv));
+  HIPCHK(h, dalloc(&S.zkey, nv));
+  HIPCHK(h, dalloc(&S.zval, nv)); HIPCHK(h, dalloc(&S.zval_tmp, nv));
   HIPCHK(h, dalloc(&S.zhead, nv)); HIPCHK(h, dalloc(&S.zend, nv));
   S.n_levels = nlev;
   S.n_sched = nsched;
@@ -717,6 +721,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   mark(h, &t0);
   HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
   launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
+  HIPCHK(h, hipMemsetAsync(S.zmark, 0, sizeof(int32_t) * (size_t)S.nv, h->st));
   launch_average_velocity(h->st, S, P.dt);        // computeAverageVelocity
   int pass_id = 0;
   bool any = false;

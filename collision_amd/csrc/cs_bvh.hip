@@ -204,7 +204,6 @@ __global__ void k_ropes(bvh_node* __restrict__ nodes, const int32_t* __restrict_
 constexpr int REFIT_THREADS = 1024;
 constexpr int REFIT_TILE = 8192;
 
-int refit_tile_count(int n) { return (n + REFIT_TILE - 1) / REFIT_TILE; }
 
 __device__ inline bool is_top(const int32_t* rmin, const int32_t* rmax, int node) {
   return (rmin[node] / REFIT_TILE) != (rmax[node] / REFIT_TILE);
@@ -230,8 +229,7 @@ __global__ void __launch_bounds__(REFIT_THREADS)
                   bvh_node* nodes, const int32_t* __restrict__ parent,
                   const int32_t* __restrict__ nright, const int32_t* __restrict__ split_right,
                   const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
-                  uint32_t* arrive, const uint8_t* __restrict__ tile_dirty) {
-  if (tile_dirty && !tile_dirty[blockIdx.x]) return;  // boxes of this tile are unchanged
+                  uint32_t* arrive) {
   const int first = blockIdx.x * REFIT_TILE;
   const int last = min(n, first + REFIT_TILE);
   for (int j = first + threadIdx.x; j < last; j += REFIT_THREADS) {
@@ -436,9 +434,7 @@ __global__ void k_leaf_of(const uint32_t* __restrict__ sorted_ids, int n, int32_
 // dirty_e[e] = element has a dirty vertex; list of dirty elements
 __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
                                                     const uint8_t* __restrict__ dirty_v, int ne,
-                                                    const int32_t* __restrict__ leaf_of,
                                                     uint8_t* __restrict__ dirty_e,
-                                                    uint8_t* __restrict__ tile_dirty,
                                                     int32_t* __restrict__ dlist,
                                                     int* __restrict__ counters) {
   __shared__ int s_cnt, s_base;
@@ -448,7 +444,6 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
     const int4 t = ev[e];
     d = dirty_v[t.x] | dirty_v[t.y] | ((t.z >= 0) ? dirty_v[t.z] : 0);
     dirty_e[e] = d ? 1 : 0;
-    if (d) tile_dirty[leaf_of[e] / REFIT_TILE] = 1;
   }
   const int slot = block_append(d, &counters[CNT_DIRTY], &s_cnt, &s_base);
   if (d) dlist[slot] = e;
@@ -486,6 +481,56 @@ __global__ void k_dirty_boxes(const int4* __restrict__ ev, const dvec4* __restri
     b.hi[d] = hi[d] + CS_BOX_EPS;
   }
   ebox[e] = b;
+  }
+}
+
+// Incremental refit by growth.  Between full refits only dirty elements
+// change their boxes; each dirty leaf gets its exact new FP32 box and then
+// grows its ancestors, stopping at the first one that already contains it.
+// Internal boxes only grow (they stay valid bounds, possibly loose, until
+// the next full refit); growth uses integer atomics on the float bit
+// patterns, so concurrent growers of one node compose.  A plain load that
+// sees containment is safe to stop on: a node's box only grows during the
+// kernel, and whoever grew it carries the growth further up.
+__device__ inline void atomic_min_f32(float* a, float v) {
+  if (v >= 0.0f) atomicMin(reinterpret_cast<int*>(a), __float_as_int(v));
+  else atomicMax(reinterpret_cast<unsigned int*>(a), __float_as_uint(v));
+}
+__device__ inline void atomic_max_f32(float* a, float v) {
+  if (v >= 0.0f) atomicMax(reinterpret_cast<int*>(a), __float_as_int(v));
+  else atomicMin(reinterpret_cast<unsigned int*>(a), __float_as_uint(v));
+}
+
+__global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebox,
+                                                    const int32_t* __restrict__ dlist,
+                                                    const int* __restrict__ counters,
+                                                    const int32_t* __restrict__ leaf_of,
+                                                    bvh_node* nodes,
+                                                    const int32_t* __restrict__ parent, int n) {
+  const int nd = counters[CNT_DIRTY];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
+    const int e = dlist[k];
+    const int leaf = (n - 1) + leaf_of[e];
+    const dbox b = ebox[e];
+    float lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = f32_down(b.lo[d]);
+      hi[d] = f32_up(b.hi[d]);
+      nodes[leaf].lo[d] = lo[d];
+      nodes[leaf].hi[d] = hi[d];
+    }
+    for (int node = parent[leaf]; node >= 0; node = parent[node]) {
+      const bvh_node c = nodes[node];
+      if (c.lo[0] <= lo[0] && c.lo[1] <= lo[1] && c.lo[2] <= lo[2] && c.hi[0] >= hi[0] &&
+          c.hi[1] >= hi[1] && c.hi[2] >= hi[2])
+        break;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (lo[d] < c.lo[d]) atomic_min_f32(&nodes[node].lo[d], lo[d]);
+        if (hi[d] > c.hi[d]) atomic_max_f32(&nodes[node].hi[d], hi[d]);
+      }
+    }
   }
 }
 
@@ -654,15 +699,20 @@ __global__ void __launch_bounds__(256) k_carry_hp(const int4* __restrict__ prev,
 }
 
 void launch_mark_dirty(hipStream_t st, DevState& S) {
-  (void)hipMemsetAsync(S.tile_dirty, 0, (size_t)S.n_tiles, st);
   hipLaunchKernelGGL(k_mark_dirty, dim3((S.ne + 255) / 256), dim3(256), 0, st, S.ev, S.dirty_v,
-                     S.ne, S.leaf_of, S.dirty_e, S.tile_dirty, S.dlist, S.counters);
+                     S.ne, S.dirty_e, S.dlist, S.counters);
 }
 
 void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
   // grid sized for every element; lanes beyond the dirty count exit
   hipLaunchKernelGGL(k_dirty_boxes, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st, S.ev, S.x_old,
                      S.av, dt, S.dlist, S.counters, S.ebox);
+}
+
+void launch_grow_dirty(hipStream_t st, DevState& S) {
+  if (S.ne < 2) return;
+  hipLaunchKernelGGL(k_grow_dirty, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st,
+                     S.ebox, S.dlist, S.counters, S.leaf_of, S.nodes, S.parent, S.ne);
 }
 
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
@@ -726,13 +776,13 @@ hipError_t build_bvh(hipStream_t st, DevState& S) {
   return hipGetLastError();
 }
 
-hipError_t refit_bvh(hipStream_t st, DevState& S, bool only_dirty_tiles) {
+hipError_t refit_bvh(hipStream_t st, DevState& S) {
   const int n = S.ne;
   if (n == 0) return hipSuccess;
   if (n > 1) CS_HIP_RET(hipMemsetAsync(S.arrive, 0, sizeof(uint32_t) * (size_t)(n - 1), st));
   hipLaunchKernelGGL(k_refit_tiles, dim3((n + REFIT_TILE - 1) / REFIT_TILE), dim3(REFIT_THREADS), 0,
                      st, S.ebox, S.sorted_ids, n, S.nodes, S.parent, S.nright, S.split_right,
-                     S.rmin, S.rmax, S.arrive, only_dirty_tiles ? S.tile_dirty : nullptr);
+                     S.rmin, S.rmax, S.arrive);
   if (n > 1 && S.n_top_start > 0)
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, n, S.parent, S.nright,
                        S.rmin, S.rmax, S.top_start, S.n_top_start, S.arrive);

@@ -43,6 +43,7 @@ enum {
   CNT_CAND64 = 16,       // box-overlap candidates of a full traversal
   CNT_OLD64 = 18,        // candidates touching dirty elements, previous boxes
   CNT_NEW64 = 20,        // candidates touching dirty elements, current boxes
+  CNT_ZLIST = 22,        // vertices in the zone member list (kept for the whole step)
   CNT_INTS = 32
 };
 
@@ -122,15 +123,17 @@ struct DevState {
   uint8_t* dirty_v = nullptr;
   uint8_t* dirty_e = nullptr;
   int32_t* dlist = nullptr;
-  uint8_t* tile_dirty = nullptr;  // refit tiles holding a dirty leaf
-  int n_tiles = 0;
   uint64_t *hkey = nullptr, *hkey_tmp = nullptr;
   uint32_t *hidx = nullptr, *hidx_tmp = nullptr;
   uint32_t *rkey = nullptr, *rkey_tmp = nullptr, *rval = nullptr, *rval_tmp = nullptr;
   int rec_cap = 0;
   // impact zones
-  uint32_t *zkey = nullptr, *zkey_tmp = nullptr, *zval = nullptr, *zval_tmp = nullptr;
-  int32_t* zsize = nullptr;
+  // zone members: vertices that took part in a union this step (zmark set,
+  // listed once in zlist); sorted by (root, vertex) for the update
+  int32_t* zmark = nullptr;
+  uint32_t* zlist = nullptr;
+  uint64_t *zkey64 = nullptr, *zkey64_tmp = nullptr;
+  uint32_t *zkey = nullptr, *zval = nullptr, *zval_tmp = nullptr;
   int32_t* zhead = nullptr;  // first sorted index of each zone
   int32_t* zend = nullptr;   // one past the last sorted index, by zone root
   // strain-limit schedule: slots in level order, slots of level L are
@@ -163,12 +166,12 @@ hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64
 void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt);
 void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt);
 hipError_t build_bvh(hipStream_t st, DevState& S);  // also finds the refit top nodes
-hipError_t refit_bvh(hipStream_t st, DevState& S, bool only_dirty_tiles = false);
-int refit_tile_count(int n);
+hipError_t refit_bvh(hipStream_t st, DevState& S);
 void launch_traverse(hipStream_t st, DevState& S, int cap);
 // incremental passes
 void launch_mark_dirty(hipStream_t st, DevState& S);
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
+void launch_grow_dirty(hipStream_t st, DevState& S);
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);
 
 // narrow phase
@@ -182,7 +185,7 @@ hipError_t apply_hits(hipStream_t st, DevState& S, int nhits);
 void launch_average_velocity(hipStream_t st, DevState& S, double dt);
 void launch_predict(hipStream_t st, DevState& S, double dt);
 void launch_uf_init(hipStream_t st, DevState& S);
-hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt);
+hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nlist);
 void launch_domain_boundary(hipStream_t st, DevState& S, const double lo[3], const double hi[3],
                             double dt, double mu, double thickness);
 void launch_final_position(hipStream_t st, DevState& S, double dt);

@@ -404,10 +404,29 @@ __device__ void uf_union(int* uf, int a, int b) {
   }
 }
 
+// zone membership: list each vertex once per step, the first time it is an
+// endpoint of a union of two distinct vertices
+struct ZoneList {
+  int* uf;
+  int* mark;
+  uint32_t* list;
+  int* count;
+};
+__device__ inline void zone_touch(const ZoneList& Z, int v) {
+  if (__hip_atomic_load(&Z.mark[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  if (atomicExch(&Z.mark[v], 1) == 0) Z.list[atomicAdd(Z.count, 1)] = (uint32_t)v;
+}
+__device__ inline void zone_union(const ZoneList& Z, int a, int b) {
+  if (a == b) return;
+  zone_touch(Z, a);
+  zone_touch(Z, b);
+  uf_union(Z.uf, a, b);
+}
+
 __global__ void __launch_bounds__(NARROW_THREADS)
     k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
              const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
-             const uint8_t* __restrict__ fixed, NarrowParams P, int* __restrict__ uf,
+             const uint8_t* __restrict__ fixed, NarrowParams P, ZoneList Z,
              hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
              int* __restrict__ counters) {
   int* err = counters + CNT_ERR;
@@ -490,12 +509,16 @@ __global__ void __launch_bounds__(NARROW_THREADS)
         bool is_pt;
         subtest_points(kind, s, P.ccd, q, &is_pt);
         // createImpZone(pts, 4): merge every pair of the four points
-        uf_union(uf, g[q[0]], g[q[1]]);
-        uf_union(uf, g[q[0]], g[q[2]]);
-        uf_union(uf, g[q[0]], g[q[3]]);
+        zone_union(Z, g[q[0]], g[q[1]]);
+        zone_union(Z, g[q[0]], g[q[2]]);
+        zone_union(Z, g[q[0]], g[q[3]]);
       }
     }
   }
+}
+
+static ZoneList zone_list(DevState& S) {
+  return ZoneList{S.uf, S.zmark, S.zlist, S.counters + CNT_ZLIST};
 }
 
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
@@ -503,13 +526,13 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
   if (npairs <= 0) return;
   hipLaunchKernelGGL(k_narrow, dim3((npairs + NARROW_THREADS - 1) / NARROW_THREADS),
                      dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
-                     S.uf, S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
+                     zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
 }
 
 // impact-zone merges of hit pairs carried from the previous pass: the points
 // of every sub-test from the pair's first hit onward
 __global__ void k_hp_unions(const int4* __restrict__ hp, int n, const int4* __restrict__ ev,
-                            int ccd, int* __restrict__ uf) {
+                            int ccd, ZoneList Z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int4 r = hp[i];
@@ -532,15 +555,16 @@ __global__ void k_hp_unions(const int4* __restrict__ hp, int n, const int4* __re
     int q[4];
     bool is_pt;
     subtest_points(kind, s, ccd, q, &is_pt);
-    uf_union(uf, g[q[0]], g[q[1]]);
-    uf_union(uf, g[q[0]], g[q[2]]);
-    uf_union(uf, g[q[0]], g[q[3]]);
+    zone_union(Z, g[q[0]], g[q[1]]);
+    zone_union(Z, g[q[0]], g[q[2]]);
+    zone_union(Z, g[q[0]], g[q[3]]);
   }
 }
 
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, S.ev, ccd, S.uf);
+  hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, S.ev, ccd,
+                     zone_list(S));
 }
 
 // parity log: (pass, lo, hi, mask) for every hit pair of the pass

@@ -159,38 +159,32 @@ __device__ inline int uf_root(const int32_t* uf, int x) {
   return x;
 }
 
-__global__ void k_zone_roots(const int32_t* __restrict__ uf, int nv, int32_t* __restrict__ zsize,
-                             uint32_t* __restrict__ root_of) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const int r = uf_root(uf, v);
-  root_of[v] = (uint32_t)r;
-  atomicAdd(&zsize[r], 1);
+// sort keys of the zone members: (root << vbits) | vertex, so one sort
+// orders zones by root and members by ascending vertex id
+__global__ void k_zone_keys(const uint32_t* __restrict__ zlist, int n, const int32_t* __restrict__ uf,
+                            int vbits, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t v = zlist[i];
+  const uint32_t r = (uint32_t)uf_root(uf, (int)v);
+  key[i] = ((uint64_t)r << vbits) | v;
+  val[i] = v;
 }
 
-__global__ void k_zone_keys(const uint32_t* __restrict__ root_of, const int32_t* __restrict__ zsize,
-                            int nv, uint32_t* __restrict__ zkey, uint32_t* __restrict__ zval) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  const uint32_t r = root_of[v];
-  zkey[v] = (zsize[r] >= 2) ? r : (uint32_t)nv;
-  zval[v] = (uint32_t)v;
-}
-
-// zone heads (first sorted index) and ends (by root)
-__global__ void __launch_bounds__(256) k_zone_heads(const uint32_t* __restrict__ zkey, int nv,
+// per sorted member: its root; zone heads (first sorted index) and ends (by root)
+__global__ void __launch_bounds__(256) k_zone_heads(const uint64_t* __restrict__ key, int n,
+                                                   int vbits, uint32_t* __restrict__ zroot,
                                                    int32_t* __restrict__ zhead,
                                                    int32_t* __restrict__ zend,
                                                    int* __restrict__ counters) {
   __shared__ int s_cnt, s_base;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool head = false;
-  if (i < nv) {
-    const uint32_t z = zkey[i];
-    if (z < (uint32_t)nv) {
-      head = (i == 0) || zkey[i - 1] != z;
-      if (i == nv - 1 || zkey[i + 1] != z) zend[z] = i + 1;
-    }
+  if (i < n) {
+    const uint32_t z = (uint32_t)(key[i] >> vbits);
+    zroot[i] = z;
+    head = (i == 0) || (uint32_t)(key[i - 1] >> vbits) != z;
+    if (i == n - 1 || (uint32_t)(key[i + 1] >> vbits) != z) zend[z] = i + 1;
   }
   const int slot = block_append(head, &counters[CNT_ZONES], &s_cnt, &s_base);
   if (head) zhead[slot] = i;
@@ -320,21 +314,26 @@ __global__ void __launch_bounds__(ZONE_LANES)
   }
 }
 
-hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt) {
-  const int nv = S.nv;
-  const int nb = (nv + 255) / 256;
-  CS_HIP_RET(hipMemsetAsync(S.zsize, 0, sizeof(int32_t) * (size_t)nv, st));
-  hipLaunchKernelGGL(k_zone_roots, dim3(nb), dim3(256), 0, st, S.uf, nv, S.zsize, S.zval_tmp);
-  hipLaunchKernelGGL(k_zone_keys, dim3(nb), dim3(256), 0, st, S.zval_tmp, S.zsize, nv, S.zkey,
-                     S.zval);
+// Every vertex of a zone with two or more members took part in a union, so
+// the union calls list the members as they go (zone_touch); only the listed
+// vertices are sorted, not the whole mesh.
+hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nlist) {
+  if (nlist <= 0) return hipSuccess;
+  const int nb = (nlist + 255) / 256;
+  int vbits = 1;
+  while (vbits < 32 && ((uint64_t)(S.nv - 1) >> vbits) != 0) ++vbits;
+  hipLaunchKernelGGL(k_zone_keys, dim3(nb), dim3(256), 0, st, S.zlist, nlist, S.uf, vbits,
+                     S.zkey64, S.zval);
   bool in_tmp = false;
-  CS_HIP_RET(radix_sort_u32(st, S.zkey, S.zval, S.zkey_tmp, S.zval_tmp, S.sort_hist, S.scan_part, nv,
-                            bits_for((uint64_t)nv), &in_tmp));
-  const uint32_t* zk = in_tmp ? S.zkey_tmp : S.zkey;
+  CS_HIP_RET(radix_sort_u64(st, S.zkey64, S.zval, S.zkey64_tmp, S.zval_tmp, S.sort_hist,
+                            S.scan_part, nlist, bits_for(((uint64_t)(S.nv - 1) << vbits) | (uint64_t)(S.nv - 1)),
+                            &in_tmp));
+  const uint64_t* zk = in_tmp ? S.zkey64_tmp : S.zkey64;
   const uint32_t* zv = in_tmp ? S.zval_tmp : S.zval;
-  hipLaunchKernelGGL(k_zone_heads, dim3(nb), dim3(256), 0, st, zk, nv, S.zhead, S.zend, S.counters);
-  hipLaunchKernelGGL(k_zone_update, dim3(std::min(nv, 2048)), dim3(ZONE_LANES), 0, st, zk, zv,
-                     S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters, S.dirty_v);
+  hipLaunchKernelGGL(k_zone_heads, dim3(nb), dim3(256), 0, st, zk, nlist, vbits, S.zkey, S.zhead,
+                     S.zend, S.counters);
+  hipLaunchKernelGGL(k_zone_update, dim3(std::min(nlist, 2048)), dim3(ZONE_LANES), 0, st, S.zkey,
+                     zv, S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters, S.dirty_v);
   return hipGetLastError();
 }
 

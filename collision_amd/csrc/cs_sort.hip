@@ -207,12 +207,113 @@ __global__ void __launch_bounds__(SORT_THREADS) k_scatter(const K* __restrict__ 
   }
 }
 
+// Small arrays (n <= SMALL_SORT_MAX): the whole LSD sort in one work-group,
+// keys and values resident in LDS for every digit pass.  Each of the 16
+// wavefronts owns a contiguous chunk and ranks it in 64-key rounds with the
+// same ballot matching as k_scatter (first a counting sweep, then the
+// placing sweep), so the result is the same stable order as the multi-kernel
+// path.  One launch replaces 5 per digit pass.
+constexpr int SMALL_SORT_MAX = 4096;
+constexpr int SMALL_THREADS = 1024;
+constexpr int SMALL_WAVES = SMALL_THREADS / 64;
+
+template <typename K>
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t bb = __ballot((d >> b) & 1u);
+    peers &= ((d >> b) & 1u) ? bb : ~bb;
+  }
+  return peers;
+}
+
+template <typename K>
+__global__ void __launch_bounds__(SMALL_THREADS) k_sort_small(K* __restrict__ keys,
+                                                              uint32_t* __restrict__ vals, int n,
+                                                              int bits) {
+  __shared__ K kb[2][SMALL_SORT_MAX];
+  __shared__ uint32_t vb[2][SMALL_SORT_MAX];
+  __shared__ uint32_t cnt[SMALL_WAVES][256];
+  __shared__ uint32_t dbase[256];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  for (int i = t; i < n; i += SMALL_THREADS) {
+    kb[0][i] = keys[i];
+    vb[0][i] = vals[i];
+  }
+  const int chunk = (((n + SMALL_WAVES - 1) / SMALL_WAVES) + 63) & ~63;
+  const int c0 = w * chunk, c1 = min(n, c0 + chunk);
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int cur = 0;
+  for (int shift = 0; shift < bits; shift += 8) {
+    for (int i = t; i < SMALL_WAVES * 256; i += SMALL_THREADS) (&cnt[0][0])[i] = 0u;
+    __syncthreads();
+    // counting sweep
+    for (int base = c0; base < c1; base += 64) {
+      const int idx = base + lane;
+      const bool valid = idx < c1;
+      const uint32_t d = valid ? (uint32_t)((kb[cur][idx] >> shift) & 255u) : 0u;
+      const uint64_t peers = digit_peers<K>(d, valid);
+      if (valid && lane == __ffsll((unsigned long long)peers) - 1) cnt[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // offsets: digit-major, then wave order inside a digit
+    if (t < 256) {
+      uint32_t run = 0;
+      for (int ww = 0; ww < SMALL_WAVES; ++ww) {
+        const uint32_t c = cnt[ww][t];
+        cnt[ww][t] = run;
+        run += c;
+      }
+      dbase[t] = run;
+    }
+    __syncthreads();
+    if (t == 0) {
+      uint32_t run = 0;
+      for (int d = 0; d < 256; ++d) {
+        const uint32_t c = dbase[d];
+        dbase[d] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    if (t < 256)
+      for (int ww = 0; ww < SMALL_WAVES; ++ww) cnt[ww][t] += dbase[t];
+    __syncthreads();
+    // placing sweep
+    for (int base = c0; base < c1; base += 64) {
+      const int idx = base + lane;
+      const bool valid = idx < c1;
+      const K k = valid ? kb[cur][idx] : (K)0;
+      const uint32_t d = valid ? (uint32_t)((k >> shift) & 255u) : 0u;
+      const uint64_t peers = digit_peers<K>(d, valid);
+      const uint32_t before = cnt[w][d];
+      if (valid) {
+        if (lane == __ffsll((unsigned long long)peers) - 1) cnt[w][d] = before + (uint32_t)__popcll(peers);
+        const uint32_t pos = before + (uint32_t)__popcll(peers & lt_mask);
+        kb[cur ^ 1][pos] = k;
+        vb[cur ^ 1][pos] = vb[cur][idx];
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int i = t; i < n; i += SMALL_THREADS) {
+    keys[i] = kb[cur][i];
+    vals[i] = vb[cur][i];
+  }
+}
+
 template <typename K>
 static hipError_t radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, K* tmp_keys,
                                   uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n,
                                   int bits, bool* result_in_tmp) {
   *result_in_tmp = false;
   if (n <= 1 || bits <= 0) return hipSuccess;
+  if (n <= SMALL_SORT_MAX) {
+    hipLaunchKernelGGL(k_sort_small<K>, dim3(1), dim3(SMALL_THREADS), 0, st, keys, vals, n, bits);
+    return hipGetLastError();
+  }
   const int nblocks = (n + SORT_TILE - 1) / SORT_TILE;
   K* kin = keys;
   uint32_t* vin = vals;
