@@ -517,13 +517,122 @@ __global__ void __launch_bounds__(NARROW_THREADS)
   }
 }
 
+// Sub-test-parallel variant for short streams (incremental passes): a group
+// of 16 lanes per pair, lane s runs sub-test s, so the dependent chain of a
+// lane is one sub-test instead of up to 15.  The group's hit mask comes from
+// one ballot; its first lane writes the hit-pair record, and every lane from
+// the first hit onward performs its own zone merges.  Hit records and counts
+// are the same as k_narrow's (record order differs, records are sorted by key
+// before use).
+constexpr int SUB_GROUP = 16;
+
+__global__ void __launch_bounds__(NARROW_THREADS)
+    k_narrow_sub(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
+                 const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
+                 const uint8_t* __restrict__ fixed, NarrowParams P, ZoneList Z,
+                 hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
+                 int* __restrict__ counters) {
+  int* err = counters + CNT_ERR;
+  const int gl = blockIdx.x * NARROW_THREADS + threadIdx.x;
+  const int i = gl / SUB_GROUP, s = gl % SUB_GROUP;
+  const int lane = threadIdx.x & 63;
+  int kind = 0, nsub = 0, A = 0, B = 0;
+  int g[6] = {0, 0, 0, 0, 0, 0};
+  if (i < npairs) {
+    const int2 pr = pairs[i];
+    A = pr.y;
+    B = pr.x;  // canonical orientation: a = hi, b = lo
+    const int4 ta = ev[A], tb = ev[B];
+    const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
+    if (tri_a && tri_b) {
+      kind = 0; nsub = 15;
+      g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
+    } else if (tri_a || tri_b) {
+      const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
+      kind = 1; nsub = 5;
+      g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
+    } else {
+      kind = 2; nsub = 1;
+      g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
+    }
+  }
+  const bool active = s < nsub;
+  int q[4] = {0, 0, 0, 0};
+  bool hit = false;
+  Sub S;
+  Contrib C;
+  if (active) {
+    bool is_pt;
+    subtest_points(kind, s, P.ccd, q, &is_pt);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      int vid = g[0];
+#pragma unroll
+      for (int k = 1; k < 6; ++k)
+        if (q[p] == k) vid = g[k];
+      const dvec4 xo = x_old[vid], vv = av[vid];
+      S.xo[p][0] = xo.x; S.xo[p][1] = xo.y; S.xo[p][2] = xo.z;
+      S.v[p][0] = vv.x; S.v[p][1] = vv.y; S.v[p][2] = vv.z;
+      S.st[p] = fixed[vid] != 0;
+      q[p] = vid;  // from here on: global vertex ids
+    }
+    hit = d_subtest(S, is_pt, P, C, err);
+  }
+  // hit records: one atomic per wavefront
+  const uint64_t hb = __ballot(hit);
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int base = 0;
+  if (lane == __ffsll((unsigned long long)hb) - 1) base = atomicAdd(&counters[CNT_HITS], __popcll(hb));
+  base = __shfl(base, __ffsll((unsigned long long)hb) - 1);
+  if (hit) {
+    const int slot = base + __popcll(hb & lt);
+    if (slot < hit_cap) {
+      hit_rec& h = hits[slot];
+      h.key = hit_key((uint32_t)B, (uint32_t)A, (uint32_t)s);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        h.v[p] = S.st[p] ? -1 : q[p];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          h.imp[p][k] = C.imp[p][k];
+          h.fric[p][k] = C.fric[p][k];
+        }
+      }
+    }
+  }
+  const int grp = lane / SUB_GROUP;
+  const int mask = (int)((hb >> (grp * SUB_GROUP)) & 0xffffull);
+  if (mask == 0) return;
+  const int first_hit = __ffs(mask) - 1;
+  if (s == 0) {
+    const int slot = atomicAdd(&counters[CNT_HIT_PAIRS], 1);
+    if (slot < hp_cap) hp[slot] = make_int4(B, A, mask, first_hit);
+    atomicAdd(&counters[CNT_HIT_SUB], __popc(mask));
+  }
+  if (P.zone_mode && active && s >= first_hit) {
+    // createImpZone(pts, 4): merge every pair of the four points
+    zone_union(Z, q[0], q[1]);
+    zone_union(Z, q[0], q[2]);
+    zone_union(Z, q[0], q[3]);
+  }
+}
+
 static ZoneList zone_list(DevState& S) {
   return ZoneList{S.uf, S.zmark, S.zlist, S.counters + CNT_ZLIST};
 }
 
+constexpr int SUB_PARALLEL_MAX_PAIRS = 1 << 16;
+
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
                    int npairs) {
   if (npairs <= 0) return;
+  if (npairs <= SUB_PARALLEL_MAX_PAIRS) {
+    const int lanes = npairs * SUB_GROUP;
+    hipLaunchKernelGGL(k_narrow_sub, dim3((lanes + NARROW_THREADS - 1) / NARROW_THREADS),
+                       dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed,
+                       P, zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
+    return;
+  }
   hipLaunchKernelGGL(k_narrow, dim3((npairs + NARROW_THREADS - 1) / NARROW_THREADS),
                      dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
                      zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
