@@ -277,7 +277,9 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     launch_grow_dirty(h->st, S);                     // leaves exact, ancestors grown
     launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
     mark(h, &T.t1);
-    launch_carry(h->st, S, prev.npairs, prev.nhits, prev.nhp);
+    // the candidate pair stream itself only feeds the exact pair count of the
+    // pass; it is carried when the caller asks for exact counts
+    launch_carry(h->st, S, exact_counts ? prev.npairs : 0, prev.nhits, prev.nhp);
     HIPCHK(h, hipGetLastError());
     int rc = read_counters(h);
     if (rc) return rc;
@@ -297,7 +299,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     carried_hits = h->h_counters[CNT_HITS];
     carried_hp = h->h_counters[CNT_HIT_PAIRS];
     carried_sub = h->h_counters[CNT_HIT_SUB];
-    npairs = carried_pairs + nq;
+    npairs = exact_counts && prev.npairs >= 0 ? carried_pairs + nq : -1;
     if (npairs > S.pair_cap) {
       const int cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
       HIPCHK(h, grow_keep(&S.pairs, carried_pairs, cap, h->st));
@@ -305,7 +307,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       HIPCHK(h, dalloc(&S.pairs_prev, cap));
       S.pair_cap = cap;
     }
-    if (nq > 0)
+    if (nq > 0 && npairs >= 0)
       HIPCHK(h, hipMemcpyAsync(S.pairs + carried_pairs, S.qpairs, sizeof(int2) * (size_t)nq,
                                hipMemcpyDeviceToDevice, h->st));
     cand = exact_counts && prev.cand != ~0ull ? prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64)
@@ -779,7 +781,8 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   for (auto& p : h->passes) {
     if (p.candidates >= 0 && R.candidates >= 0) R.candidates += p.candidates;
     else R.candidates = -1;
-    R.narrow_pairs += p.narrow_pairs;
+    if (p.narrow_pairs >= 0 && R.narrow_pairs >= 0) R.narrow_pairs += p.narrow_pairs;
+    else R.narrow_pairs = -1;
     R.hit_pairs += p.hit_pairs;
     R.hit_subtests += p.hit_subtests;
     R.ms_broad += p.ms_broad;

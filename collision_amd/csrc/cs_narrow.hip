@@ -1,9 +1,8 @@
 // Narrow phase for gfx950: static proximity and continuous collision tests
 // over the compacted candidate stream, FP64.
 //
-// One lane per element pair.  The pair's vertices (x_old, avgVel, static
-// flag) are staged once into LDS (thread-major SoA, conflict-free) and every
-// sub-test gathers its four points from there.  Sub-tests follow the
+// One lane per (pair, sub-test): each lane gathers its sub-test's four
+// points (x_old, avgVel, static flag) and runs it.  Sub-tests follow the
 // reference exactly:
 //   proximity: TriToTri / TriToBond / BondToBond (dcollid3d.cpp:485-627),
 //              PointToTri (778-922), EdgeToEdge (643-776) with h = thickness
@@ -423,118 +422,25 @@ __device__ inline void zone_union(const ZoneList& Z, int a, int b) {
   uf_union(Z.uf, a, b);
 }
 
-__global__ void __launch_bounds__(NARROW_THREADS)
-    k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
-             const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
-             const uint8_t* __restrict__ fixed, NarrowParams P, ZoneList Z,
-             hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
-             int* __restrict__ counters) {
-  int* err = counters + CNT_ERR;
-  __shared__ double lx[6 * 3 * NARROW_THREADS];
-  __shared__ double lv[6 * 3 * NARROW_THREADS];
-  __shared__ uint8_t lst[6 * NARROW_THREADS];
-  const int t = threadIdx.x;
-  const int i = blockIdx.x * NARROW_THREADS + t;
-  if (i >= npairs) return;
-  const int2 pr = pairs[i];
-  const int A = pr.y, B = pr.x;  // canonical orientation: a = hi, b = lo
-  const int4 ta = ev[A], tb = ev[B];
-  const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
-  int kind, nloc, nsub;
-  int g[6];
-  if (tri_a && tri_b) {
-    kind = 0; nloc = 6; nsub = 15;
-    g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
-  } else if (tri_a || tri_b) {
-    const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
-    kind = 1; nloc = 5; nsub = 5;
-    g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
-  } else {
-    kind = 2; nloc = 4; nsub = 1;
-    g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
-  }
-  for (int k = 0; k < nloc; ++k) {
-    const dvec4 xo = x_old[g[k]];
-    const dvec4 vv = av[g[k]];
-    lx[(k * 3 + 0) * NARROW_THREADS + t] = xo.x;
-    lx[(k * 3 + 1) * NARROW_THREADS + t] = xo.y;
-    lx[(k * 3 + 2) * NARROW_THREADS + t] = xo.z;
-    lv[(k * 3 + 0) * NARROW_THREADS + t] = vv.x;
-    lv[(k * 3 + 1) * NARROW_THREADS + t] = vv.y;
-    lv[(k * 3 + 2) * NARROW_THREADS + t] = vv.z;
-    lst[k * NARROW_THREADS + t] = fixed[g[k]];
-  }
-  int mask = 0, first_hit = -1;
-  for (int s = 0; s < nsub; ++s) {
-    int q[4];
-    bool is_pt;
-    subtest_points(kind, s, P.ccd, q, &is_pt);
-    Sub S;
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        S.xo[p][k] = lx[(q[p] * 3 + k) * NARROW_THREADS + t];
-        S.v[p][k] = lv[(q[p] * 3 + k) * NARROW_THREADS + t];
-      }
-      S.st[p] = lst[q[p] * NARROW_THREADS + t] != 0;
-    }
-    Contrib C;
-    if (d_subtest(S, is_pt, P, C, err)) {
-      mask |= 1 << s;
-      if (first_hit < 0) first_hit = s;
-      const int slot = atomicAdd(&counters[CNT_HITS], 1);
-      if (slot < hit_cap) {
-        hit_rec& h = hits[slot];
-        h.key = hit_key((uint32_t)B, (uint32_t)A, (uint32_t)s);
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-          h.v[p] = S.st[p] ? -1 : g[q[p]];
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            h.imp[p][k] = C.imp[p][k];
-            h.fric[p][k] = C.fric[p][k];
-          }
-        }
-      }
-    }
-  }
-  if (mask) {
-    const int slot = atomicAdd(&counters[CNT_HIT_PAIRS], 1);
-    if (slot < hp_cap) hp[slot] = make_int4(B, A, mask, first_hit);
-    atomicAdd(&counters[CNT_HIT_SUB], __popc(mask));
-    if (P.zone_mode) {
-      for (int s = first_hit; s < nsub; ++s) {
-        int q[4];
-        bool is_pt;
-        subtest_points(kind, s, P.ccd, q, &is_pt);
-        // createImpZone(pts, 4): merge every pair of the four points
-        zone_union(Z, g[q[0]], g[q[1]]);
-        zone_union(Z, g[q[0]], g[q[2]]);
-        zone_union(Z, g[q[0]], g[q[3]]);
-      }
-    }
-  }
-}
-
-// Sub-test-parallel variant for short streams (incremental passes): a group
-// of 16 lanes per pair, lane s runs sub-test s, so the dependent chain of a
-// lane is one sub-test instead of up to 15.  The group's hit mask comes from
-// one ballot; its first lane writes the hit-pair record, and every lane from
-// the first hit onward performs its own zone merges.  Hit records and counts
-// are the same as k_narrow's (record order differs, records are sorted by key
-// before use).
+// One group of 16 lanes per pair, lane s runs sub-test s (15 for two
+// triangles, 5 for a triangle and a bond, 1 for two bonds), so the dependent
+// chain of a lane is one sub-test.  The group's hit mask comes from one
+// ballot; its first lane writes the hit-pair record, and every lane from the
+// first hit onward performs its own zone merges (the cumulative-status
+// rule).  Hit records take one atomic per wavefront; their order does not
+// matter (they are sorted by key before use).
 constexpr int SUB_GROUP = 16;
 
 __global__ void __launch_bounds__(NARROW_THREADS)
-    k_narrow_sub(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
+    k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
                  const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
                  const uint8_t* __restrict__ fixed, NarrowParams P, ZoneList Z,
                  hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
                  int* __restrict__ counters) {
   int* err = counters + CNT_ERR;
-  const int gl = blockIdx.x * NARROW_THREADS + threadIdx.x;
-  const int i = gl / SUB_GROUP, s = gl % SUB_GROUP;
+  const int64_t gl = (int64_t)blockIdx.x * NARROW_THREADS + threadIdx.x;
+  const int64_t i = gl / SUB_GROUP;
+  const int s = (int)(gl % SUB_GROUP);
   const int lane = threadIdx.x & 63;
   int kind = 0, nsub = 0, A = 0, B = 0;
   int g[6] = {0, 0, 0, 0, 0, 0};
@@ -621,19 +527,11 @@ static ZoneList zone_list(DevState& S) {
   return ZoneList{S.uf, S.zmark, S.zlist, S.counters + CNT_ZLIST};
 }
 
-constexpr int SUB_PARALLEL_MAX_PAIRS = 1 << 16;
-
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
                    int npairs) {
   if (npairs <= 0) return;
-  if (npairs <= SUB_PARALLEL_MAX_PAIRS) {
-    const int lanes = npairs * SUB_GROUP;
-    hipLaunchKernelGGL(k_narrow_sub, dim3((lanes + NARROW_THREADS - 1) / NARROW_THREADS),
-                       dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed,
-                       P, zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
-    return;
-  }
-  hipLaunchKernelGGL(k_narrow, dim3((npairs + NARROW_THREADS - 1) / NARROW_THREADS),
+  const int64_t lanes = (int64_t)npairs * SUB_GROUP;
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((lanes + NARROW_THREADS - 1) / NARROW_THREADS)),
                      dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
                      zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
 }

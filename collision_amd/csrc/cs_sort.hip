@@ -268,13 +268,21 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_sort_small(K* __restrict__ ke
       dbase[t] = run;
     }
     __syncthreads();
-    if (t == 0) {
-      uint32_t run = 0;
-      for (int d = 0; d < 256; ++d) {
-        const uint32_t c = dbase[d];
-        dbase[d] = run;
-        run += c;
+    if (w == 0) {  // exclusive scan of the 256 digit totals by one wavefront
+      const uint32_t c0 = dbase[4 * lane], c1 = dbase[4 * lane + 1], c2 = dbase[4 * lane + 2],
+                     c3 = dbase[4 * lane + 3];
+      const uint32_t sum = c0 + c1 + c2 + c3;
+      uint32_t incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
       }
+      const uint32_t ex = incl - sum;
+      dbase[4 * lane] = ex;
+      dbase[4 * lane + 1] = ex + c0;
+      dbase[4 * lane + 2] = ex + c0 + c1;
+      dbase[4 * lane + 3] = ex + c0 + c1 + c2;
     }
     __syncthreads();
     if (t < 256)

@@ -82,7 +82,8 @@ typedef struct cs_pass_info {
     int64_t candidates;      /* box-overlapping element pairs; -1 in an incremental
                                 pass unless log_pairs is set (the exact count needs
                                 an extra query of the changed elements) */
-    int64_t narrow_pairs;    /* pairs whose sub-tests ran */
+    int64_t narrow_pairs;    /* candidate pairs after the early rejects; -1 in an
+                                incremental pass unless log_pairs is set */
     int64_t hit_pairs;       /* pairs whose narrow phase reported contact */
     int64_t hit_subtests;
     int64_t tested_pairs;    /* pairs the narrow-phase kernel evaluated in this pass
@@ -107,7 +108,8 @@ typedef struct cs_step_report {
     int32_t degenerate_edges;
     int32_t strain_levels;   /* schedule levels the strain sweeps evaluated (sum) */
     int64_t candidates;      /* sum over passes; -1 when a pass reports -1 */
-    int64_t narrow_pairs, hit_pairs, hit_subtests;
+    int64_t narrow_pairs;    /* sum over passes; -1 when a pass reports -1 */
+    int64_t hit_pairs, hit_subtests;
     double ms_total, ms_broad, ms_narrow, ms_response, ms_zone, ms_strain;
     int64_t strain_slots;    /* element visits the strain sweeps evaluated (sum) */
 } cs_step_report;
