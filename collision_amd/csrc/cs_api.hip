@@ -31,10 +31,15 @@ std::string g_create_error;
 struct PrevPass {
   bool valid = false;  // previous pass was a CCD / zone pass of this step
   int npairs = 0, nhits = 0, nhp = 0;
+  int nq = 0;  // re-queried pairs (incremental pass)
   unsigned long long cand = 0;
 };
 
 }  // namespace
+
+struct PassTimes {
+  hipEvent_t e0, e1, e2, e3, t0, t1, n0, n1;
+};
 
 struct cs_solver {
   cs_params P{};
@@ -46,6 +51,7 @@ struct cs_solver {
   int has_collision = 0;
   int* h_counters = nullptr;  // pinned mirror of the counter block
   std::vector<cs_pass_info> passes;
+  std::vector<PassTimes> ptimes;  // events of each pass, read after the step
   int64_t plog_rows = 0;
   std::vector<hipEvent_t> events;
   size_t ev_used = 0;
@@ -96,7 +102,7 @@ void free_dev(DevState& S) {
   dfree(S.zkey); dfree(S.zval); dfree(S.zval_tmp);
   dfree(S.zhead); dfree(S.zend);
   dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters);
-  dfree(S.shead); dfree(S.sbody); dfree(S.sflag);
+  dfree(S.shead); dfree(S.snl); dfree(S.sbody); dfree(S.sflag);
   dfree(S.lvl_active);
   S = DevState();
 }
@@ -193,10 +199,6 @@ int device_error(cs_solver* h) {
   return CS_OK;
 }
 
-struct PassTimes {
-  hipEvent_t e0, e1, e2, e3, t0, t1, n0, n1;
-};
-
 unsigned long long c64(const cs_solver* h, int slot) {
   return *(const unsigned long long*)(h->h_counters + slot);
 }
@@ -212,6 +214,24 @@ void swap_ptr(T*& a, T*& b) {
   T* t = a;
   a = b;
   b = t;
+}
+
+// grow the hit and hit-pair buffers to hold nhits / nhp records, keeping
+// the first keep_hits / keep_hp entries
+hipError_t grow_hit_buffers(cs_solver* h, int nhits, int nhp, int keep_hits, int keep_hp) {
+  DevState& S = h->S;
+  if (nhits > S.hit_cap) {
+    CS_HIP_RET(alloc_hits(S, nhits + nhits / 2 + 1024, keep_hits, h->st));
+    CS_HIP_RET(alloc_sort_hist(S));
+  }
+  if (nhp > S.hp_cap) {
+    const int cap = nhp + nhp / 2 + 1024;
+    CS_HIP_RET(grow_keep(&S.hp, keep_hp, cap, h->st));
+    dfree(S.hp_prev);
+    CS_HIP_RET(dalloc(&S.hp_prev, cap));
+    S.hp_cap = cap;
+  }
+  return hipSuccess;
 }
 
 // One broad + narrow + response pass.  *any_hit = some pair reported contact.
@@ -240,9 +260,18 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
   HIPCHK(h, hipMemsetAsync(S.counters + CNT_DIRTY, 0, sizeof(int), h->st));
   HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 6 * sizeof(int), h->st));
-  int npairs = 0, carried_hits = 0, carried_hp = 0, carried_sub = 0;
+  NarrowParams NP;
+  NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
+  NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
+  NP.ccd = ccd;
+  NP.zone_mode = (kind == CS_PASS_ZONE);
+  NP.log_pairs = P.log_pairs;
+  NP.pass_id = pass_id;
+  int npairs = 0, carried_hits = 0, carried_hp = 0, carried_sub = 0, carried_pairs = 0;
   unsigned long long cand = 0;
   int nq = 0;
+  int nhits = 0, nhp = 0;
+  int narrow_n = 0;
   if (!incremental) {
     HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
     launch_element_boxes(h->st, S, ccd, P.dt);
@@ -265,7 +294,30 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 2 * sizeof(int), h->st));
     }
     cand = c64(h, CNT_CAND64);
+    mark(h, &T.e1);
+    narrow_n = npairs;
+    for (;;) {
+      mark(h, &T.n0);
+      launch_narrow(h->st, S, NP, S.pairs, npairs, nullptr, 0);
+      mark(h, &T.n1);
+      HIPCHK(h, hipGetLastError());
+      int rc = read_counters(h);
+      if (rc) return rc;
+      nhits = h->h_counters[CNT_HITS];
+      nhp = h->h_counters[CNT_HIT_PAIRS];
+      if (nhits <= S.hit_cap && nhp <= S.hp_cap) break;
+      HIPCHK(h, grow_hit_buffers(h, nhits, nhp, 0, 0));
+      if ((rc = set_counter(h, CNT_HITS, 0))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_PAIRS, 0))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_SUB, 0))) return rc;
+    }
   } else {
+    // One host synchronisation per pass: the narrow phase reads the number
+    // of re-queried pairs from device memory, the counts carried over from
+    // the previous pass are snapshotted on the device before the narrow
+    // phase appends to them, and overflows are detected afterwards (the
+    // narrow phase only appends records and performs idempotent zone
+    // merges, so it can be run again).
     launch_mark_dirty(h->st, S);
     HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
     mark(h, &T.t0);
@@ -280,25 +332,52 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // the candidate pair stream itself only feeds the exact pair count of the
     // pass; it is carried when the caller asks for exact counts
     launch_carry(h->st, S, exact_counts ? prev.npairs : 0, prev.nhits, prev.nhp);
-    HIPCHK(h, hipGetLastError());
-    int rc = read_counters(h);
-    if (rc) return rc;
-    nq = h->h_counters[CNT_QPAIRS];
-    if (nq > S.qpair_cap) {  // grow and query again
-      S.qpair_cap = nq + nq / 2 + 1024;
-      dfree(S.qpairs);
-      HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
-      HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
-      HIPCHK(h, hipMemsetAsync(S.counters + CNT_NEW64, 0, 2 * sizeof(int), h->st));
-      launch_query_dirty(h->st, S, 1);
-      rc = read_counters(h);
+    HIPCHK(h, hipMemcpyAsync(S.counters + CNT_SNAP, S.counters, sizeof(int) * 4,
+                             hipMemcpyDeviceToDevice, h->st));
+    mark(h, &T.e1);
+    // impact-zone merges of the carried hit pairs (union order does not
+    // change the partition); their count is read on the device
+    if (kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, prev.nhp, 1);
+    // grid for a guess of the re-queried pair count; pairs beyond it are
+    // handled after the count is known
+    const int guess = std::min(S.qpair_cap, std::max(4096, 2 * prev.nq + 1024));
+    for (int attempt = 0;; ++attempt) {
+      mark(h, &T.n0);
+      launch_narrow(h->st, S, NP, S.qpairs, guess, S.counters + CNT_QPAIRS, S.qpair_cap);
+      mark(h, &T.n1);
+      HIPCHK(h, hipGetLastError());
+      int rc = read_counters(h);
       if (rc) return rc;
       nq = h->h_counters[CNT_QPAIRS];
+      if (nq > guess && nq <= S.qpair_cap) {  // the rest of the stream
+        launch_narrow(h->st, S, NP, S.qpairs + guess, nq - guess, nullptr, 0);
+        mark(h, &T.n1);
+        HIPCHK(h, hipGetLastError());
+        if ((rc = read_counters(h))) return rc;
+      }
+      carried_pairs = h->h_counters[CNT_SNAP + CNT_PAIRS];
+      carried_hits = h->h_counters[CNT_SNAP + CNT_HITS];
+      carried_hp = h->h_counters[CNT_SNAP + CNT_HIT_PAIRS];
+      carried_sub = h->h_counters[CNT_SNAP + CNT_HIT_SUB];
+      nhits = h->h_counters[CNT_HITS];
+      nhp = h->h_counters[CNT_HIT_PAIRS];
+      const bool q_over = nq > S.qpair_cap;
+      if (!q_over && nhits <= S.hit_cap && nhp <= S.hp_cap) break;
+      if (attempt > 4) return fail(h, CS_ERR_INTERNAL, "narrow-phase buffers keep overflowing");
+      if (q_over) {  // grow the query stream and query again
+        S.qpair_cap = nq + nq / 2 + 1024;
+        dfree(S.qpairs);
+        HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
+        HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
+        HIPCHK(h, hipMemsetAsync(S.counters + CNT_NEW64, 0, 2 * sizeof(int), h->st));
+        launch_query_dirty(h->st, S, 1);
+      }
+      HIPCHK(h, grow_hit_buffers(h, nhits, nhp, carried_hits, carried_hp));
+      if ((rc = set_counter(h, CNT_HITS, carried_hits))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_PAIRS, carried_hp))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_SUB, carried_sub))) return rc;
     }
-    const int carried_pairs = h->h_counters[CNT_PAIRS];
-    carried_hits = h->h_counters[CNT_HITS];
-    carried_hp = h->h_counters[CNT_HIT_PAIRS];
-    carried_sub = h->h_counters[CNT_HIT_SUB];
+    narrow_n = nq;
     npairs = exact_counts && prev.npairs >= 0 ? carried_pairs + nq : -1;
     if (npairs > S.pair_cap) {
       const int cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
@@ -313,48 +392,6 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     cand = exact_counts && prev.cand != ~0ull ? prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64)
                                               : ~0ull;
   }
-  mark(h, &T.e1);
-  NarrowParams NP;
-  NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
-  NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
-  NP.ccd = ccd;
-  NP.zone_mode = (kind == CS_PASS_ZONE);
-  NP.log_pairs = P.log_pairs;
-  NP.pass_id = pass_id;
-  // impact-zone merges of the carried hit pairs (union order does not
-  // change the partition; run first so the member count read after the
-  // narrow phase is complete)
-  if (incremental && kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, carried_hp, 1);
-  const int2* narrow_in = incremental ? S.qpairs : S.pairs;
-  const int narrow_n = incremental ? nq : npairs;
-  int nhits = 0, nhp = 0;
-  for (;;) {
-    mark(h, &T.n0);
-    launch_narrow(h->st, S, NP, narrow_in, narrow_n);
-    mark(h, &T.n1);
-    HIPCHK(h, hipGetLastError());
-    int rc = read_counters(h);
-    if (rc) return rc;
-    nhits = h->h_counters[CNT_HITS];
-    nhp = h->h_counters[CNT_HIT_PAIRS];
-    if (nhits <= S.hit_cap && nhp <= S.hp_cap) break;
-    // grow (keeping carried entries) and run the narrow phase again; it only
-    // appends records and performs idempotent zone unions
-    if (nhits > S.hit_cap) {
-      HIPCHK(h, alloc_hits(S, nhits + nhits / 2 + 1024, carried_hits, h->st));
-      HIPCHK(h, alloc_sort_hist(S));
-    }
-    if (nhp > S.hp_cap) {
-      const int cap = nhp + nhp / 2 + 1024;
-      HIPCHK(h, grow_keep(&S.hp, carried_hp, cap, h->st));
-      dfree(S.hp_prev);
-      HIPCHK(h, dalloc(&S.hp_prev, cap));
-      S.hp_cap = cap;
-    }
-    if ((rc = set_counter(h, CNT_HITS, carried_hits))) return rc;
-    if ((rc = set_counter(h, CNT_HIT_PAIRS, carried_hp))) return rc;
-    if ((rc = set_counter(h, CNT_HIT_SUB, carried_sub))) return rc;
-  }
   mark(h, &T.e2);
   HIPCHK(h, apply_hits(h->st, S, nhits));
   if (kind == CS_PASS_ZONE) {
@@ -363,9 +400,6 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark(h, &T.e3);
-  int rc = read_counters(h);
-  if (rc) return rc;
-  if (P.log_pairs) h->plog_rows = std::min<int64_t>(h->h_counters[CNT_LOG], S.plog_cap);
   cs_pass_info pi{};
   pi.kind = kind;
   pi.index = index;
@@ -375,18 +409,20 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   pi.incremental = incremental ? 1 : 0;
   pi.hit_pairs = nhp;
   pi.hit_subtests = h->h_counters[CNT_HIT_SUB];
-  pi.ms_broad = elapsed(T.e0, T.e1);
-  pi.ms_narrow = elapsed(T.e1, T.e2);
-  pi.ms_response = elapsed(T.e2, T.e3);
-  pi.ms_traverse_kernel = elapsed(T.t0, T.t1);
-  pi.ms_narrow_kernel = elapsed(T.n0, T.n1);
+  if (P.log_pairs) {  // the pair log is read after every pass
+    int rc = read_counters(h);
+    if (rc) return rc;
+    h->plog_rows = std::min<int64_t>(h->h_counters[CNT_LOG], S.plog_cap);
+  }
   h->passes.push_back(pi);
+  h->ptimes.push_back(T);  // durations are read once the step has finished
   *any_hit = nhp > 0;
   if (ccd) {
     h->prev.valid = true;
     h->prev.npairs = npairs;
     h->prev.nhits = nhits;
     h->prev.nhp = nhp;
+    h->prev.nq = incremental ? nq : 0;
     h->prev.cand = cand;
   }
   return device_error(h);
@@ -552,6 +588,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
       if (level[e] >= 0) slot_of[e] = fill[level[e]]++;
   }
   std::vector<StrainHead> shead(nslot);
+  std::vector<int4> snl(nslot, make_int4(-1, -1, -1, -1));
   std::vector<StrainBody> sbody(nslot);
   std::memset(shead.data(), 0xff, sizeof(StrainHead) * nslot);
   std::memset(sbody.data(), 0, sizeof(StrainBody) * nslot);
@@ -566,21 +603,21 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
       const int vv[3] = {ev[e].x, ev[e].y, tri ? ev[e].z : -1};
       const int s = slot_of[e];
       StrainHead& hd = shead[s];
+      int nl[3] = {-1, -1, -1};
       for (int i = 0; i < 3; ++i) {
         hd.v[i] = vv[i];
-        hd.nl[i] = -1;
         hd.ns[i] = -1;
       }
       for (int i = 0; i < np; ++i) {
         const int ns = nxt_slot[vv[i]];
         if (ns >= 0) {
           hd.ns[i] = 4 * ns + nxt_pos[vv[i]];  // byte of the next slot's flag word
-          hd.nl[i] = shead[ns].level;
+          nl[i] = shead[ns].level;
         }
       }
+      snl[s] = make_int4(nl[0], nl[1], nl[2], -1);
       hd.level = level[e];
       hd.elem = e;
-      hd.pad = 0;
       for (int i = 0; i < np; ++i) {
         nxt_slot[vv[i]] = s;
         nxt_pos[vv[i]] = i;
@@ -631,7 +668,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   S.n_levels = nlev;
   S.n_sched = nsched;
   HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1));
-  HIPCHK(h, dalloc(&S.shead, nslot)); HIPCHK(h, dalloc(&S.sbody, nslot));
+  HIPCHK(h, dalloc(&S.shead, nslot)); HIPCHK(h, dalloc(&S.snl, nslot));
+  HIPCHK(h, dalloc(&S.sbody, nslot));
   HIPCHK(h, dalloc(&S.sflag, nslot)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
   S.plog_cap = 1 << 20;
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
@@ -643,6 +681,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.fixed, fixed.data(), nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.lvl_ptr, lptr.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.shead, shead.data(), sizeof(StrainHead) * nslot, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.snl, snl.data(), sizeof(int4) * nslot, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.sbody, sbody.data(), sizeof(StrainBody) * nslot, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.sflag, 0, sizeof(uint32_t) * nslot));
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
@@ -715,6 +754,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   const cs_params& P = h->P;
   cs_step_report R{};
   h->passes.clear();
+  h->ptimes.clear();
   h->plog_rows = 0;
   h->ev_used = 0;
   h->has_collision = 0;
@@ -748,7 +788,6 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
       if (zi >= P.max_zone_passes) { status = CS_WARN_ZONE_CAP; break; }
       rc = run_pass(h, CS_PASS_ZONE, zi, pass_id++, &is_collision);
       if (rc) { status = rc; break; }
-      R.n_zones = h->h_counters[CNT_ZONES];
       R.n_zone_passes++;
       zi++;
     }
@@ -766,6 +805,16 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   mark(h, &t_end);
   rc = read_counters(h);
   if (rc) return rc;
+  if (R.n_zone_passes) R.n_zones = h->h_counters[CNT_ZONES];
+  for (size_t i = 0; i < h->passes.size() && i < h->ptimes.size(); ++i) {
+    cs_pass_info& pi = h->passes[i];
+    const PassTimes& T = h->ptimes[i];
+    pi.ms_broad = elapsed(T.e0, T.e1);
+    pi.ms_narrow = elapsed(T.e1, T.e2);
+    pi.ms_response = elapsed(T.e2, T.e3);
+    pi.ms_traverse_kernel = elapsed(T.t0, T.t1);
+    pi.ms_narrow_kernel = elapsed(T.n0, T.n1);
+  }
   if (status == CS_OK || status == CS_WARN_ZONE_CAP) {
     R.strain_sweeps = h->h_counters[CNT_STRAIN_SWEEPS];
     R.strain_edges = h->h_counters[CNT_STRAIN_EDGES];

@@ -44,6 +44,7 @@ enum {
   CNT_OLD64 = 18,        // candidates touching dirty elements, previous boxes
   CNT_NEW64 = 20,        // candidates touching dirty elements, current boxes
   CNT_ZLIST = 22,        // vertices in the zone member list (kept for the whole step)
+  CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
   CNT_INTS = 32
 };
 
@@ -55,20 +56,19 @@ struct NarrowParams {
   int pass_id;
 };
 
-// All device memory of one solver handle.
 // One slot of the strain-limit schedule (one non-rigid element).  The head
-// (48 B) is what every visit of the slot reads; the body (128 B) is read only
-// when the slot is evaluated.  Rest lengths are written at assembly; x_old of
-// the vertices and the x_old edge lengths are packed once per step.
+// (32 B) is what a visit of the slot reads first; the body (128 B) is read
+// only when the slot is evaluated; the next-visit levels only when a dirty
+// vertex's next visit is not on the following level.  Rest lengths are
+// written at assembly; x_old of the vertices and the x_old edge lengths are
+// packed once per step.
 struct alignas(16) StrainHead {
   int v[3];    // vertex ids; v[2] = -1 for a bond
-  int nl[3];   // level of each vertex's next visit (-1: none)
   int ns[3];   // byte index (4 * slot + position) of the next visit's dirty flag (-1: none)
   int level;
   int elem;    // element id
-  int pad;
 };
-static_assert(sizeof(StrainHead) == 48, "StrainHead layout");
+static_assert(sizeof(StrainHead) == 32, "StrainHead layout");
 struct alignas(16) StrainBody {
   double x[9];     // x_old of vertices 0, 1, 2 (vertex 2 unused for a bond)
   double lold[3];  // x_old lengths of edges (0,1), (1,2), (2,0)
@@ -77,6 +77,7 @@ struct alignas(16) StrainBody {
 };
 static_assert(sizeof(StrainBody) == 128, "StrainBody layout");
 
+// All device memory of one solver handle.
 struct DevState {
   int nv = 0, nt = 0, nb = 0, ne = 0;
   // vertices
@@ -140,6 +141,7 @@ struct DevState {
   // [lvl_ptr[L], lvl_ptr[L+1])
   int32_t* lvl_ptr = nullptr;
   StrainHead* shead = nullptr;
+  int4* snl = nullptr;             // per slot: level of each vertex's next visit (-1: none)
   StrainBody* sbody = nullptr;
   uint32_t* sflag = nullptr;       // per slot: bytes 0-2 vertex dirty, byte 3 pre-check active
   uint8_t* lvl_active = nullptr;   // level holds an active element (pre-check)
@@ -176,7 +178,7 @@ void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, 
 
 // narrow phase
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
-                   int npairs);
+                   int npairs, const int* npairs_dev, int cap);
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd);
 void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id);
 

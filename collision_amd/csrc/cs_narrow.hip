@@ -431,14 +431,13 @@ __device__ inline void zone_union(const ZoneList& Z, int a, int b) {
 // matter (they are sorted by key before use).
 constexpr int SUB_GROUP = 16;
 
-__global__ void __launch_bounds__(NARROW_THREADS)
-    k_narrow(const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
-                 const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
-                 const uint8_t* __restrict__ fixed, NarrowParams P, ZoneList Z,
-                 hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
-                 int* __restrict__ counters) {
+__device__ __attribute__((always_inline)) inline void narrow_lane(
+    int64_t gl, const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
+    const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
+    const uint8_t* __restrict__ fixed, const NarrowParams& P, const ZoneList& Z,
+    hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
+    int* __restrict__ counters) {
   int* err = counters + CNT_ERR;
-  const int64_t gl = (int64_t)blockIdx.x * NARROW_THREADS + threadIdx.x;
   const int64_t i = gl / SUB_GROUP;
   const int s = (int)(gl % SUB_GROUP);
   const int lane = threadIdx.x & 63;
@@ -523,25 +522,42 @@ __global__ void __launch_bounds__(NARROW_THREADS)
   }
 }
 
+// npairs_dev != nullptr: the pair count is read on the device (clamped to
+// cap); the grid covers the first npairs pairs of it
+__global__ void __launch_bounds__(NARROW_THREADS, 3)
+    k_narrow(const int2* __restrict__ pairs, int npairs, const int* __restrict__ npairs_dev,
+             int cap, const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
+             const dvec4* __restrict__ av, const uint8_t* __restrict__ fixed, NarrowParams P,
+             ZoneList Z, hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp,
+             int hp_cap, int* __restrict__ counters) {
+  const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
+  const int64_t gl = (int64_t)blockIdx.x * NARROW_THREADS + threadIdx.x;
+  if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
+  narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
+}
+
 static ZoneList zone_list(DevState& S) {
   return ZoneList{S.uf, S.zmark, S.zlist, S.counters + CNT_ZLIST};
 }
 
+// npairs_dev: the pair count is read on the device and only its first
+// npairs pairs are covered (the caller checks the count afterwards)
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
-                   int npairs) {
+                   int npairs, const int* npairs_dev, int cap) {
   if (npairs <= 0) return;
   const int64_t lanes = (int64_t)npairs * SUB_GROUP;
-  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((lanes + NARROW_THREADS - 1) / NARROW_THREADS)),
-                     dim3(NARROW_THREADS), 0, st, pairs, npairs, S.ev, S.x_old, S.av, S.fixed, P,
-                     zone_list(S), S.hits, S.hit_cap, S.hp, S.hp_cap, S.counters);
+  const int64_t blocks = (lanes + NARROW_THREADS - 1) / NARROW_THREADS;
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)blocks), dim3(NARROW_THREADS), 0, st, pairs, npairs,
+                     npairs_dev, cap, S.ev, S.x_old, S.av, S.fixed, P, zone_list(S), S.hits,
+                     S.hit_cap, S.hp, S.hp_cap, S.counters);
 }
 
 // impact-zone merges of hit pairs carried from the previous pass: the points
 // of every sub-test from the pair's first hit onward
-__global__ void k_hp_unions(const int4* __restrict__ hp, int n, const int4* __restrict__ ev,
-                            int ccd, ZoneList Z) {
+__global__ void k_hp_unions(const int4* __restrict__ hp, int nmax, const int* __restrict__ n_dev,
+                            const int4* __restrict__ ev, int ccd, ZoneList Z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= min(nmax, *n_dev)) return;
   const int4 r = hp[i];
   const int A = r.y, B = r.x;
   const int4 ta = ev[A], tb = ev[B];
@@ -568,10 +584,12 @@ __global__ void k_hp_unions(const int4* __restrict__ hp, int n, const int4* __re
   }
 }
 
+// the records carried into this pass: their count is the hit-pair counter
+// before the narrow phase appends (n is an upper bound)
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, S.ev, ccd,
-                     zone_list(S));
+  hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n,
+                     S.counters + CNT_HIT_PAIRS, S.ev, ccd, zone_list(S));
 }
 
 // parity log: (pass, lo, hi, mask) for every hit pair of the pass

@@ -439,7 +439,7 @@ void launch_uf_init(hipStream_t st, DevState& S) {
 // ---------------------------------------------------------------------------
 // strain limiting
 // edge test with the x_old length given (it is fixed during the step)
-__device__ inline bool edge_over_limit_l(const double* o0, const double* o1, const dvec4& a0,
+__device__ __attribute__((always_inline)) inline bool edge_over_limit_l(const double* o0, const double* o1, const dvec4& a0,
                                          const dvec4& a1, double len_old, double len0, double dt,
                                          bool* degenerate) {
   const double xc0[3] = {o0[0] + dt * a0.x, o0[1] + dt * a0.y, o0[2] + dt * a0.z};
@@ -461,13 +461,16 @@ __device__ inline bool edge_over_limit_l(const double* o0, const double* o1, con
 // later visits of written ("dirty") vertices are evaluated, in level order.
 //
 // Dirty state travels along each vertex's chain of visits: a slot that
-// evaluates a dirty vertex sets the flag byte of that vertex in the slot of
-// its next visit (and the LDS flag of that slot's level), so every later
-// element holding the vertex is evaluated, as in a sequential sweep.  A
-// slot's flag word (bytes 0-2 dirty vertices, byte 3 pre-check active) and
-// head are one coalesced read per level; only evaluated slots read their
-// body (x_old, lengths) and velocities.  Levels whose LDS flag is clear are
-// skipped without touching memory.
+// evaluates a dirty vertex marks that vertex in the slot of its next visit,
+// so every later element holding the vertex is evaluated, as in a
+// sequential sweep.  Almost every next visit is on the following level; such
+// marks go to an LDS table for that level, the rest to the slot's flag word
+// in memory (bytes 0-2 dirty vertices, byte 3 pre-check active) with an LDS
+// bit telling that level to re-read its flag words.  The heads and flag
+// words of the following level and of the next pre-check-active level are
+// loaded one level ahead, so a level's critical path is: LDS marks -> body and
+// velocities of the evaluated slots (one memory round trip) -> arithmetic ->
+// barrier.  Levels with nothing to do are skipped using LDS level bits.
 
 // x_old of the slot vertices and edge lengths, once per step
 __global__ void k_strain_pack(const StrainHead* __restrict__ head, StrainBody* __restrict__ body,
@@ -517,7 +520,10 @@ __global__ void k_strain_check(const StrainHead* __restrict__ head,
 }
 
 constexpr int STRAIN_THREADS = 1024;
-constexpr int STRAIN_LDS_LEVELS = 65536;  // levels whose flags fit in LDS
+constexpr int STRAIN_TAB = STRAIN_THREADS;  // slots of a level on the LDS-mark path
+constexpr int STRAIN_LDS_LEVELS = 65536;    // levels whose bits fit in LDS
+constexpr int STRAIN_LW = STRAIN_LDS_LEVELS / 32;
+constexpr int STRAIN_LP_MAX = 8192;         // levels whose slot offsets are kept in LDS
 
 struct StrainCounts {
   int* flag;
@@ -525,12 +531,22 @@ struct StrainCounts {
   int* degen;
 };
 
-// one edge of the sweep: average the two velocities when over the limit
-__device__ inline bool strain_edge(const double* xa, const double* xb, dvec4& a, dvec4& b,
-                                   double lold, double L0, double dt, int ia, int ib, dvec4* av,
-                                   const StrainCounts& C) {
-  bool deg;
-  if (!edge_over_limit_l(xa, xb, a, b, lold, L0, dt, &deg)) return false;
+__device__ inline bool bit_get(const uint32_t* b, int i) { return (b[i >> 5] >> (i & 31)) & 1u; }
+__device__ inline void bit_set(uint32_t* b, int i) { atomicOr(&b[i >> 5], 1u << (i & 31)); }
+// first set bit at index >= from (n if none)
+__device__ inline int bit_next(const uint32_t* b, int from, int n) {
+  int i = from;
+  while (i < n) {
+    const uint32_t w = b[i >> 5] >> (i & 31);
+    if (w) return min(n, i + (int)__builtin_ctz(w));
+    i = (i | 31) + 1;
+  }
+  return n;
+}
+
+// average the two velocities of an edge found over the limit
+__device__ inline void strain_average(dvec4& a, dvec4& b, bool deg, int ia, int ib, dvec4* av,
+                                      const StrainCounts& C) {
   double vt[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
   vt[0] = 0.5 * vt[0];
   vt[1] = 0.5 * vt[1];
@@ -543,67 +559,119 @@ __device__ inline bool strain_edge(const double* xa, const double* xb, dvec4& a,
   *C.flag = 1;
   atomicAdd(C.edges, 1);
   if (deg) atomicAdd(C.degen, 1);
-  return true;
 }
+
+// where a dirty vertex's next visit is marked
+struct StrainMarks {
+  uint32_t* tab_next;   // LDS marks of level L + 1 (fast path), indexed from nb
+  int nb, ne;           // slot range of level L + 1; ne == nb when L + 1 is not on the fast path
+  uint8_t* sflag_b;     // flag bytes in memory (other levels)
+  const int4* snl;      // next-visit levels (for marks in memory)
+  uint32_t* actb;       // LDS: level has work
+  uint32_t* gscb;       // LDS: level must re-read its flag words
+  bool use_lds;
+};
 
 // evaluate one slot's edges in order, (0,1), (1,2), (2,0), with the current
-// velocities (reduceSuperelast's per-element loop, dcollid.cpp:515-560);
-// returns whether the slot was evaluated
-__device__ inline bool strain_slot(const StrainHead& h, uint32_t f, const StrainBody* body, int s,
-                                   uint8_t* sflag_b, dvec4* av, uint8_t* act, bool use_lds,
-                                   double dt, const StrainCounts& C) {
+// velocities (reduceSuperelast's per-element loop, dcollid.cpp:515-560)
+__device__ __attribute__((always_inline)) inline void strain_eval(const StrainHead& h, uint32_t f, const StrainBody* body,
+                                   int s, int next_level, dvec4* av, const StrainMarks M,
+                                   double dt, const StrainCounts C) {
   bool d0 = (f & 0xffu) != 0, d1 = (f & 0xff00u) != 0, d2 = (f & 0xff0000u) != 0;
-  if (f == 0u) return false;  // not pre-check active and no dirty vertex
   const bool tri = h.v[2] >= 0;
-  const StrainBody& r = body[s];
+  // the whole body and all velocities are loaded before the first store
+  // (the stores to av would otherwise order later body loads behind them)
+  const StrainBody r = body[s];
   dvec4 a0 = av[h.v[0]], a1 = av[h.v[1]], a2{};
   if (tri) a2 = av[h.v[2]];
-  if (strain_edge(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, h.v[0], h.v[1], av, C))
+  bool g;
+  if (edge_over_limit_l(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, &g)) {
+    strain_average(a0, a1, g, h.v[0], h.v[1], av, C);
     d0 = d1 = true;
+  }
   if (tri) {
-    if (strain_edge(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, h.v[1], h.v[2], av, C))
+    if (edge_over_limit_l(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, &g)) {
+      strain_average(a1, a2, g, h.v[1], h.v[2], av, C);
       d1 = d2 = true;
-    if (strain_edge(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, h.v[2], h.v[0], av, C))
+    }
+    if (edge_over_limit_l(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, &g)) {
+      strain_average(a2, a0, g, h.v[2], h.v[0], av, C);
       d2 = d0 = true;
+    }
   }
   // every later visit of a dirty vertex must be evaluated
-  if (d0 && h.ns[0] >= 0) {
-    sflag_b[h.ns[0]] = 1;
-    if (use_lds) act[h.nl[0]] = 1;
+  const bool dk[3] = {d0, d1, d2};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ns = h.ns[k];
+    if (!dk[k] || ns < 0) continue;
+    const int slot = ns >> 2;
+    if (slot >= M.nb && slot < M.ne) {  // next visit on the following level
+      atomicOr(&M.tab_next[slot - M.nb], 1u << (8 * (ns & 3)));
+      if (M.use_lds) bit_set(M.actb, next_level);
+    } else {
+      M.sflag_b[ns] = 1;
+      if (M.use_lds) {
+        const int4 nl4 = M.snl[s];
+        const int nl = k == 0 ? nl4.x : (k == 1 ? nl4.y : nl4.z);
+        bit_set(M.actb, nl);
+        bit_set(M.gscb, nl);
+      }
+    }
   }
-  if (d1 && h.ns[1] >= 0) {
-    sflag_b[h.ns[1]] = 1;
-    if (use_lds) act[h.nl[1]] = 1;
-  }
-  if (d2 && h.ns[2] >= 0) {
-    sflag_b[h.ns[2]] = 1;
-    if (use_lds) act[h.nl[2]] = 1;
-  }
-  return true;
 }
 
-// One work-group walks the levels in order; lane t owns slots b + t,
-// b + t + STRAIN_THREADS, ... of a level.
+// a head held in registers as two int4 (v0 v1 v2 ns0 | ns1 ns2 level elem),
+// so a prefetched head never needs an addressable copy
+struct HeadRegs {
+  int4 a, b;
+};
+__device__ inline HeadRegs load_head(const StrainHead* __restrict__ head, int s) {
+  const int4* p = reinterpret_cast<const int4*>(head + s);
+  return HeadRegs{p[0], p[1]};
+}
+__device__ inline void store_head(StrainHead& h, const HeadRegs& r) {
+  h.v[0] = r.a.x; h.v[1] = r.a.y; h.v[2] = r.a.z;
+  h.ns[0] = r.a.w; h.ns[1] = r.b.x; h.ns[2] = r.b.y;
+  h.level = r.b.z; h.elem = r.b.w;
+}
+
+// One work-group walks the levels in order.  On the fast path (levels of at
+// most STRAIN_TAB slots) lane t owns slot b + t; larger levels are walked
+// with a stride and always read their flag words from memory.
+template <bool LP_LDS>
 __global__ void __launch_bounds__(STRAIN_THREADS)
     k_strain_sweep(const int32_t* __restrict__ lvl_ptr, int n_levels,
-                   const StrainHead* __restrict__ head, const StrainBody* __restrict__ body,
-                   uint32_t* sflag, const uint8_t* __restrict__ lvl_active_g, dvec4* av,
-                   double dt, int* __restrict__ counters) {
-  __shared__ uint8_t act[STRAIN_LDS_LEVELS];
+                   const StrainHead* __restrict__ head, const int4* __restrict__ snl,
+                   const StrainBody* __restrict__ body, uint32_t* sflag,
+                   const uint8_t* __restrict__ lvl_active_g, dvec4* av, double dt,
+                   int* __restrict__ counters) {
+  __shared__ uint32_t actb[STRAIN_LW], preb[STRAIN_LW], gscb[STRAIN_LW];
+  __shared__ uint32_t tab[2][STRAIN_TAB];
+  __shared__ int32_t lp[STRAIN_LP_MAX + 1];
+  // slots of the current level to evaluate, packed so that the FP64 work
+  // of a level lands in as few wavefronts as possible
+  __shared__ StrainHead wk_head[STRAIN_TAB];
+  __shared__ int32_t wk_slot[STRAIN_TAB];
+  __shared__ uint32_t wk_f[STRAIN_TAB];
+  __shared__ int s_nwork[2];
   __shared__ int s_flag, s_edges, s_degen, s_state, s_levels, s_evals;
-  if (threadIdx.x == 0) {
+  const int t = threadIdx.x;
+  if (t == 0) {
     s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweep needed
     s_flag = 0;
     s_edges = 0;
     s_degen = 0;
     s_levels = 0;
     s_evals = 0;
+    s_nwork[0] = 0;
+    s_nwork[1] = 0;
   }
   __syncthreads();
   const int state = s_state;
   if (state < 0) return;
   if (state == 0) {  // nothing over the limit: this sweep is a no-op and ends the loop
-    if (threadIdx.x == 0) {
+    if (t == 0) {
       counters[CNT_STRAIN_FLAG] = -1;
       counters[CNT_STRAIN_EDGES] = 0;
       atomicAdd(&counters[CNT_STRAIN_SWEEPS], 1);
@@ -612,25 +680,118 @@ __global__ void __launch_bounds__(STRAIN_THREADS)
   }
   const StrainCounts C{&s_flag, &s_edges, &s_degen};
   const bool use_lds = n_levels <= STRAIN_LDS_LEVELS;
-  if (use_lds)
-    for (int L = threadIdx.x; L < n_levels; L += STRAIN_THREADS) act[L] = lvl_active_g[L];
-  __syncthreads();
-  uint8_t* sflag_b = reinterpret_cast<uint8_t*>(sflag);
-  int evals = 0;
-  for (int L = 0; L < n_levels; ++L) {
-    if (use_lds && !act[L]) continue;  // uniform: flags change only before a barrier
-    const int b = lvl_ptr[L], e_end = lvl_ptr[L + 1];
-    for (int s = b + threadIdx.x; s < e_end; s += STRAIN_THREADS) {
-      const uint32_t f = sflag[s];  // last written before an earlier barrier
-      const StrainHead h = head[s];
-      evals += strain_slot(h, f, body, s, sflag_b, av, act, use_lds, dt, C);
+  const int nw = use_lds ? (n_levels + 31) / 32 : 0;
+  for (int w = t; w < nw; w += STRAIN_THREADS) {
+    uint32_t bits = 0;
+    for (int j = 0; j < 32; ++j) {
+      const int L = 32 * w + j;
+      if (L < n_levels && lvl_active_g[L]) bits |= 1u << j;
     }
-    if (threadIdx.x == 0) ++s_levels;
+    actb[w] = bits;
+    preb[w] = bits;
+    gscb[w] = 0u;
+  }
+  for (int i = t; i < 2 * STRAIN_TAB; i += STRAIN_THREADS) (&tab[0][0])[i] = 0u;
+  if (LP_LDS)
+    for (int i = t; i <= n_levels; i += STRAIN_THREADS) lp[i] = lvl_ptr[i];
+  __syncthreads();
+  // slot offset of level i (from LDS when the offsets fit there)
+  auto lvp = [&](int i) -> int {
+    if constexpr (LP_LDS) return lp[i];
+    else return lvl_ptr[i];
+  };
+  uint8_t* sflag_b = reinterpret_cast<uint8_t*>(sflag);
+  // prefetched head + flag word of this lane's slot: level L + 1 (pn) and
+  // the next pre-check-active level (pq)
+  HeadRegs pn{}, pq{};
+  uint32_t pnf = 0u, pqf = 0u;
+  int pn_level = -1, pq_level = -1;
+  int cur = 0, evals = 0;
+  int L = use_lds ? bit_next(actb, 0, n_levels) : 0;
+  while (L < n_levels) {
+    const int b = lvp(L), e = lvp(L + 1);
+    const bool fast = use_lds && (e - b) <= STRAIN_TAB;
+    // slots of the following level (mark target of the fast path)
+    int nb = 0, ne = 0, ne2 = 0;
+    if (L + 1 < n_levels) {
+      nb = e;
+      ne2 = lvp(L + 2);
+      ne = (use_lds && ne2 - nb <= STRAIN_TAB) ? ne2 : nb;
+    }
+    const StrainMarks M{tab[cur ^ 1], nb, ne, sflag_b, snl, actb, gscb, use_lds};
+    if (fast) {
+      HeadRegs h{};
+      uint32_t f = 0u;
+      const bool ok = b + t < e;
+      if (pn_level == L) {
+        h = pn;
+        f = pnf;
+      } else if (pq_level == L) {
+        h = pq;
+        f = pqf;
+      } else if (ok) {
+        h = load_head(head, b + t);
+        f = sflag[b + t];
+      }
+      if (ok && bit_get(gscb, L)) f = sflag[b + t];  // marks written to memory since the prefetch
+      const uint32_t m = tab[cur][t];
+      f |= m;
+      if (m) tab[cur][t] = 0u;
+      if (ok && f != 0u) {
+        const int k = atomicAdd(&s_nwork[cur], 1);
+        store_head(wk_head[k], h);
+        wk_slot[k] = b + t;
+        wk_f[k] = f;
+      }
+      // prefetch the following level and the next pre-check-active level;
+      // issued after this level's own loads were consumed, so no wait of
+      // this level covers them
+      pn_level = -1;
+      if (ne > nb) {
+        pn_level = L + 1;
+        if (nb + t < ne) {
+          pn = load_head(head, nb + t);
+          pnf = sflag[nb + t];
+        }
+      }
+      const int P = bit_next(preb, L + 2, n_levels);
+      if (P < n_levels && P != pq_level) {
+        const int pb = lvp(P), pe = lvp(P + 1);
+        pq_level = -1;
+        if (pe - pb <= STRAIN_TAB) {
+          pq_level = P;
+          if (pb + t < pe) {
+            pq = load_head(head, pb + t);
+            pqf = sflag[pb + t];
+          }
+        }
+      }
+      __syncthreads();
+      const int nwork = s_nwork[cur];
+      if (t == 0) s_nwork[cur ^ 1] = 0;  // next level's list; last read before the previous barrier
+      for (int i = t; i < nwork; i += STRAIN_THREADS) {
+        strain_eval(wk_head[i], wk_f[i], body, wk_slot[i], L + 1, av, M, dt, C);
+        ++evals;
+      }
+    } else {
+      // large level (or no LDS bits): flag words from memory, heads loaded here
+      for (int s = b + t; s < e; s += STRAIN_THREADS) {
+        const uint32_t f = sflag[s];
+        if (f == 0u) continue;
+        const StrainHead h = head[s];
+        strain_eval(h, f, body, s, L + 1, av, M, dt, C);
+        ++evals;
+      }
+      pn_level = -1;
+    }
+    if (t == 0) ++s_levels;
     __syncthreads();
+    cur ^= 1;
+    L = use_lds ? bit_next(actb, L + 1, n_levels) : L + 1;
   }
   if (evals) atomicAdd(&s_evals, evals);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     counters[CNT_STRAIN_EDGES] = s_edges;
     counters[CNT_STRAIN_DEGEN] += s_degen;
     counters[CNT_STRAIN_SWEEPS] += 1;
@@ -650,9 +811,14 @@ hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, 
     CS_HIP_RET(hipMemsetAsync(S.lvl_active, 0, (size_t)S.n_levels, st));
     hipLaunchKernelGGL(k_strain_check, dim3((n + 255) / 256), dim3(256), 0, st, S.shead, S.sbody,
                        S.av, dt, n, S.sflag, S.lvl_active, S.counters);
-    hipLaunchKernelGGL(k_strain_sweep, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
-                       S.n_levels, S.shead, S.sbody, S.sflag, S.lvl_active, S.av, dt,
-                       S.counters);
+    if (S.n_levels <= STRAIN_LP_MAX)
+      hipLaunchKernelGGL(k_strain_sweep<true>, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
+                         S.n_levels, S.shead, S.snl, S.sbody, S.sflag, S.lvl_active, S.av, dt,
+                         S.counters);
+    else
+      hipLaunchKernelGGL(k_strain_sweep<false>, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
+                         S.n_levels, S.shead, S.snl, S.sbody, S.sflag, S.lvl_active, S.av, dt,
+                         S.counters);
   }
   return hipGetLastError();
 }
