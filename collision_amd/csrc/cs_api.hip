@@ -91,7 +91,9 @@ void free_dev(DevState& S) {
   dfree(S.fixed); dfree(S.has); dfree(S.uf);
   dfree(S.ev); dfree(S.eflags); dfree(S.len0); dfree(S.ebox);
   dfree(S.bounds); dfree(S.mcode); dfree(S.mid); dfree(S.mcode_tmp); dfree(S.mid_tmp);
-  dfree(S.sort_hist); dfree(S.nodes); S.leaves = nullptr; dfree(S.parent); dfree(S.nright); dfree(S.split_right);
+  dfree(S.sort_hist); dfree(S.nodes); dfree(S.parent); dfree(S.nright); dfree(S.split_right);
+  dfree(S.lpre); dfree(S.kleft); dfree(S.kright); dfree(S.kparent); dfree(S.krmin);
+  dfree(S.krmax); dfree(S.pre);
   dfree(S.rmin); dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
@@ -641,12 +643,15 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.mcode, ne)); HIPCHK(h, dalloc(&S.mid, ne));
   HIPCHK(h, dalloc(&S.mcode_tmp, ne)); HIPCHK(h, dalloc(&S.mid_tmp, ne));
   HIPCHK(h, dalloc(&S.nodes, 2 * ne - 1));
-  S.leaves = S.nodes + (ne - 1);
   HIPCHK(h, dalloc(&S.parent, 2 * ne - 1));
-  HIPCHK(h, dalloc(&S.nright, std::max(ne - 1, 1)));
+  HIPCHK(h, dalloc(&S.nright, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.lpre, ne));
+  HIPCHK(h, dalloc(&S.kleft, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.kright, std::max(ne - 1, 1)));
+  HIPCHK(h, dalloc(&S.kparent, 2 * ne - 1)); HIPCHK(h, dalloc(&S.pre, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.krmin, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.krmax, std::max(ne - 1, 1)));
   HIPCHK(h, dalloc(&S.split_right, std::max(ne - 1, 1)));
-  HIPCHK(h, dalloc(&S.rmin, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.rmax, std::max(ne - 1, 1)));
-  HIPCHK(h, dalloc(&S.arrive, std::max(ne - 1, 1))); HIPCHK(h, dalloc(&S.top_start, std::max(ne - 1, 1)));
+  HIPCHK(h, dalloc(&S.rmin, 2 * ne - 1)); HIPCHK(h, dalloc(&S.rmax, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.arrive, 2 * ne - 1)); HIPCHK(h, dalloc(&S.top_start, std::max(ne - 1, 1)));
   S.pair_cap = (int)std::min<int64_t>(3ll * ne + 4096, INT32_MAX / 2);
   HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
   HIPCHK(h, dalloc(&S.pairs_prev, S.pair_cap));

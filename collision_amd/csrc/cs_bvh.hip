@@ -9,8 +9,9 @@
 //     reference's min/max_{static,moving}_coord, dcollid.cpp:852-932).
 //   * k_bounds / k_morton : 30-bit Morton codes of FP32 box centres.
 //   * (radix sort of codes, cs_sort.hip)
-//   * k_karras        : radix tree over the sorted codes (index tie-break),
-//     k_ropes         : escape links for stackless depth-first traversal.
+//   * k_karras        : radix tree over the sorted codes (index tie-break);
+//     k_preorder / k_permute : the tree laid out in depth-first pre-order
+//     with escape links (ropes) for stackless traversal.
 //   * k_refit_tiles / k_refit_top : bottom-up FP32 boxes rounded outward,
 //     so every FP64 overlap is also an FP32 overlap.
 //   * k_traverse      : one lane per leaf in Morton order; it starts at its
@@ -142,15 +143,15 @@ __device__ inline int lcp_delta(const uint32_t* __restrict__ codes, int n, int i
   return __clz(a ^ b);
 }
 
-// Karras (2012) radix tree: internal nodes 0..n-2, leaf j is node n-1+j.
-// The node splitting after leaf g records its second child in
-// split_right[g]: that child is what depth-first order visits right after
-// any subtree whose last leaf is g (every split position 0..n-2 belongs to
-// exactly one internal node).
-__global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __restrict__ nodes,
-                         int32_t* __restrict__ parent, int32_t* __restrict__ nright,
-                         int32_t* __restrict__ split_right, int32_t* __restrict__ rmin,
-                         int32_t* __restrict__ rmax) {
+// Karras (2012) radix tree, built in construction arrays: internal nodes
+// 0..n-2, leaf j is node n-1+j.  The node splitting after leaf g records its
+// second child in split_right[g]: that child is what depth-first order
+// visits right after any subtree whose last leaf is g (every split position
+// 0..n-2 belongs to exactly one internal node).
+__global__ void k_karras(const uint32_t* __restrict__ codes, int n, int32_t* __restrict__ kleft,
+                         int32_t* __restrict__ kright, int32_t* __restrict__ kparent,
+                         int32_t* __restrict__ split_right, int32_t* __restrict__ krmin,
+                         int32_t* __restrict__ krmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (lcp_delta(codes, n, i, i + 1) - lcp_delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -172,24 +173,65 @@ __global__ void k_karras(const uint32_t* __restrict__ codes, int n, bvh_node* __
   const int lo = min(i, j), hi = max(i, j);
   const int left = (lo == gamma) ? (n - 1) + gamma : gamma;
   const int right = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
-  nodes[i].left = left;
-  nright[i] = right;
+  kleft[i] = left;
+  kright[i] = right;
   split_right[gamma] = right;
-  rmin[i] = lo;
-  rmax[i] = hi;
-  parent[left] = i;
-  parent[right] = i;
-  if (i == 0) parent[0] = -1;
+  krmin[i] = lo;
+  krmax[i] = hi;
+  kparent[left] = i;
+  kparent[right] = i;
+  if (i == 0) kparent[0] = -1;
 }
 
-// rope of internal node i: the second child of the node splitting after the
-// subtree's last leaf (none after the last leaf)
-__global__ void k_ropes(bvh_node* __restrict__ nodes, const int32_t* __restrict__ rmax,
-                        const int32_t* __restrict__ split_right, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  const int last = rmax[i];
-  nodes[i].rope = (last == n - 1) ? -1 : split_right[last];
+// Depth-first (pre-order) position of every construction node: the nodes
+// before x are its ancestors and the subtrees left of its path, which hold
+// exactly the leaves before x's first leaf a, so pre(x) = 2a + (number of
+// left turns from the root to x).  The traversal layout is pre-order: a
+// node's first child is the next record and its rope points forward, so a
+// walk through the tree touches memory in increasing address order.
+__global__ void k_preorder(int n, const int32_t* __restrict__ kleft,
+                           const int32_t* __restrict__ kparent, const int32_t* __restrict__ krmin,
+                           int32_t* __restrict__ pre) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= 2 * n - 1) return;
+  const int a = (x < n - 1) ? krmin[x] : x - (n - 1);
+  int left_turns = 0;
+  for (int c = x, p = kparent[x]; p >= 0; c = p, p = kparent[p])
+    if (kleft[p] == c) ++left_turns;
+  pre[x] = 2 * a + left_turns;
+}
+
+// scatter the construction tree into the pre-order arrays
+__global__ void k_permute(int n, const int32_t* __restrict__ pre, const int32_t* __restrict__ kright,
+                          const int32_t* __restrict__ kparent, const int32_t* __restrict__ split_right,
+                          const int32_t* __restrict__ krmin, const int32_t* __restrict__ krmax,
+                          const uint32_t* __restrict__ sorted_ids, bvh_node* __restrict__ nodes,
+                          int32_t* __restrict__ parent, int32_t* __restrict__ nright,
+                          int32_t* __restrict__ rmin, int32_t* __restrict__ rmax,
+                          int32_t* __restrict__ lpre, int32_t* __restrict__ enode) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= 2 * n - 1) return;
+  const int p = pre[x];
+  const bool internal = x < n - 1;
+  const int last = internal ? krmax[x] : x - (n - 1);
+  const int rk = (last == n - 1) ? -1 : split_right[last];
+  nodes[p].rope = rk < 0 ? -1 : pre[rk];
+  parent[p] = kparent[x] < 0 ? -1 : pre[kparent[x]];
+  if (internal) {
+    nodes[p].left = -1;
+    nright[p] = pre[kright[x]];
+    rmin[p] = krmin[x];
+    rmax[p] = krmax[x];
+  } else {
+    const int j = x - (n - 1);
+    const int e = (int)sorted_ids[j];
+    nodes[p].left = e;
+    nright[p] = -1;
+    rmin[p] = j;
+    rmax[p] = j;
+    lpre[j] = p;
+    enode[e] = p;
+  }
 }
 
 // Refit.  Every internal node covers a contiguous range [rmin, rmax] of
@@ -209,9 +251,10 @@ __device__ inline bool is_top(const int32_t* rmin, const int32_t* rmax, int node
   return (rmin[node] / REFIT_TILE) != (rmax[node] / REFIT_TILE);
 }
 
+// the first child of an internal node is the next record (pre-order)
 __device__ inline void merge_children(bvh_node* nodes, const int32_t* __restrict__ nright,
                                       int node) {
-  const bvh_node A = nodes[nodes[node].left];
+  const bvh_node A = nodes[node + 1];
   const bvh_node B = nodes[nright[node]];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -227,7 +270,7 @@ __device__ inline uint32_t wg_arrive(uint32_t* p) {
 __global__ void __launch_bounds__(REFIT_THREADS)
     k_refit_tiles(const dbox* __restrict__ ebox, const uint32_t* __restrict__ sorted_ids, int n,
                   bvh_node* nodes, const int32_t* __restrict__ parent,
-                  const int32_t* __restrict__ nright, const int32_t* __restrict__ split_right,
+                  const int32_t* __restrict__ nright, const int32_t* __restrict__ lpre,
                   const int32_t* __restrict__ rmin, const int32_t* __restrict__ rmax,
                   uint32_t* arrive) {
   const int first = blockIdx.x * REFIT_TILE;
@@ -235,17 +278,14 @@ __global__ void __launch_bounds__(REFIT_THREADS)
   for (int j = first + threadIdx.x; j < last; j += REFIT_THREADS) {
     const int e = (int)sorted_ids[j];
     const dbox b = ebox[e];
-    bvh_node lf;
+    const int leaf = lpre[j];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      lf.lo[d] = f32_down(b.lo[d]);
-      lf.hi[d] = f32_up(b.hi[d]);
+      nodes[leaf].lo[d] = f32_down(b.lo[d]);
+      nodes[leaf].hi[d] = f32_up(b.hi[d]);
     }
-    lf.left = e;
-    lf.rope = (j == n - 1) ? -1 : split_right[j];
-    nodes[(n - 1) + j] = lf;
     if (n == 1) continue;
-    int node = parent[(n - 1) + j];
+    int node = parent[leaf];
     while (node >= 0 && !is_top(rmin, rmax, node)) {
       if (wg_arrive(&arrive[node]) == 0u) break;  // first arrival: sibling not done
       merge_children(nodes, nright, node);
@@ -263,10 +303,8 @@ __global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool start = false;
-  if (i < n - 1 && is_top(rmin, rmax, i)) {
-    const int l = nodes[i].left, r = nright[i];
-    start = !(l < n - 1 && is_top(rmin, rmax, l)) && !(r < n - 1 && is_top(rmin, rmax, r));
-  }
+  if (i < 2 * n - 1 && nodes[i].left < 0 && is_top(rmin, rmax, i))
+    start = !is_top(rmin, rmax, i + 1) && !is_top(rmin, rmax, nright[i]);
   int slot = -1;
   if (start) slot = atomicAdd(&s_cnt, 1);
   __syncthreads();
@@ -276,7 +314,7 @@ __global__ void k_mark_top(const bvh_node* __restrict__ nodes, const int32_t* __
 }
 
 __global__ void __launch_bounds__(1024)
-    k_refit_top(bvh_node* nodes, int n, const int32_t* __restrict__ parent,
+    k_refit_top(bvh_node* nodes, const int32_t* __restrict__ parent,
                 const int32_t* __restrict__ nright, const int32_t* __restrict__ rmin,
                 const int32_t* __restrict__ rmax, const int32_t* __restrict__ starts, int nstarts,
                 uint32_t* arrive) {
@@ -286,9 +324,8 @@ __global__ void __launch_bounds__(1024)
     node = parent[node];
     while (node >= 0) {
       // arrivals needed = number of top children (the others are done)
-      const int l = nodes[node].left, r = nright[node];
-      const uint32_t need = (uint32_t)((l < n - 1 && is_top(rmin, rmax, l)) ? 1 : 0) +
-                            (uint32_t)((r < n - 1 && is_top(rmin, rmax, r)) ? 1 : 0);
+      const uint32_t need = (uint32_t)(is_top(rmin, rmax, node + 1) ? 1 : 0) +
+                            (uint32_t)(is_top(rmin, rmax, nright[node]) ? 1 : 0);
       if (wg_arrive(&arrive[node]) + 1u < need) break;
       merge_children(nodes, nright, node);
       node = parent[node];
@@ -340,6 +377,7 @@ constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flush
 
 __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
+                                                           const int32_t* __restrict__ lpre,
                                                            const dbox* __restrict__ ebox,
                                                            const int4* __restrict__ ev,
                                                            const uint8_t* __restrict__ eflags,
@@ -352,11 +390,10 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
   __shared__ int s_base;
   const int t = threadIdx.x;
   const int i = blockIdx.x * TRAV_THREADS + t;
-  const int leaf0 = n - 1;  // index of leaf 0
   unsigned long long cand = 0;
   int nbuf = 0;
   if (i < n && n > 1) {
-    const bvh_node me = nodes[leaf0 + i];
+    const bvh_node me = nodes[lpre[i]];
     const int e = me.left;
     const dbox mb = ebox[e];
     const int4 mt = ev[e];
@@ -369,8 +406,8 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
         nd = node.rope;
         continue;
       }
-      if (nd < leaf0) {
-        nd = node.left;
+      if (node.left < 0) {  // internal: its first child is the next record
+        ++nd;
         continue;
       }
       nd = node.rope;
@@ -426,10 +463,6 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
 // Candidate counts stay exact: count = previous - (pairs touching dirty
 // elements under the previous boxes) + (the same under the new boxes).
 
-__global__ void k_leaf_of(const uint32_t* __restrict__ sorted_ids, int n, int32_t* __restrict__ leaf_of) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) leaf_of[sorted_ids[j]] = j;
-}
 
 // dirty_e[e] = element has a dirty vertex; list of dirty elements
 __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
@@ -504,13 +537,13 @@ __device__ inline void atomic_max_f32(float* a, float v) {
 __global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebox,
                                                     const int32_t* __restrict__ dlist,
                                                     const int* __restrict__ counters,
-                                                    const int32_t* __restrict__ leaf_of,
+                                                    const int32_t* __restrict__ enode,
                                                     bvh_node* nodes,
                                                     const int32_t* __restrict__ parent, int n) {
   const int nd = counters[CNT_DIRTY];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
     const int e = dlist[k];
-    const int leaf = (n - 1) + leaf_of[e];
+    const int leaf = enode[e];
     const dbox b = ebox[e];
     float lo[3], hi[3];
 #pragma unroll
@@ -542,14 +575,13 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
     const bvh_node* __restrict__ nodes, const dbox* __restrict__ ebox,
     const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
     const uint8_t* __restrict__ dirty_e, const int32_t* __restrict__ dlist,
-    const int32_t* __restrict__ leaf_of, int n, int emit, int2* __restrict__ qpairs, int qcap,
+    const int32_t* __restrict__ enode, int n, int emit, int2* __restrict__ qpairs, int qcap,
     int* __restrict__ counters) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
   __shared__ int s_base;
   __shared__ unsigned long long s_w[TRAV_THREADS / 64];
   __shared__ int s_off[TRAV_THREADS];
   const int t = threadIdx.x;
-  const int leaf0 = n - 1;
   const int ndirty = counters[CNT_DIRTY];
   const int rounds = (ndirty + (int)(gridDim.x * TRAV_THREADS) - 1) / (int)(gridDim.x * TRAV_THREADS);
   for (int round = 0; round < rounds; ++round) {
@@ -558,8 +590,8 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
   int nbuf = 0;
   if (k < ndirty && n > 1) {
     const int e = dlist[k];
-    const int si = leaf_of[e];
-    const bvh_node me = nodes[leaf0 + si];
+    const int si = enode[e];  // pre-order index of e's leaf (increasing with Morton order)
+    const bvh_node me = nodes[si];
     const dbox mb = ebox[e];
     const int4 mt = ev[e];
     const uint8_t mf = eflags[e];
@@ -570,11 +602,11 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
         nd = node.rope;
         continue;
       }
-      if (nd < leaf0) {
-        nd = node.left;
+      if (node.left < 0) {  // internal: its first child is the next record
+        ++nd;
         continue;
       }
-      const int j = nd - leaf0;
+      const int j = nd;
       nd = node.rope;
       if (j == si) continue;
       const int e2 = node.left;
@@ -756,17 +788,32 @@ hipError_t build_bvh(hipStream_t st, DevState& S) {
                             30, &in_tmp));
   S.sorted_codes = in_tmp ? S.mcode_tmp : S.mcode;
   S.sorted_ids = in_tmp ? S.mid_tmp : S.mid;
-  hipLaunchKernelGGL(k_leaf_of, dim3((n + 255) / 256), dim3(256), 0, st, S.sorted_ids, n, S.leaf_of);
   S.n_top_start = 0;
-  if (n > 1) {
+  if (n == 1) {  // a single leaf: node 0
+    bvh_node leaf{};
+    leaf.left = 0;
+    leaf.rope = -1;
+    const int32_t zero = 0, none = -1;
+    CS_HIP_RET(hipMemcpyAsync(S.nodes, &leaf, sizeof(leaf), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipMemcpyAsync(S.parent, &none, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipMemcpyAsync(S.lpre, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipMemcpyAsync(S.leaf_of, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipMemcpyAsync(S.rmin, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipMemcpyAsync(S.rmax, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    CS_HIP_RET(hipStreamSynchronize(st));  // host sources above live on this stack frame
+  } else {
+    const int nn = 2 * n - 1;
     // every split position is written by k_karras; -1 (end) is a safe default
     CS_HIP_RET(hipMemsetAsync(S.split_right, 0xff, sizeof(int32_t) * (size_t)(n - 1), st));
     hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.sorted_codes, n,
-                       S.nodes, S.parent, S.nright, S.split_right, S.rmin, S.rmax);
-    hipLaunchKernelGGL(k_ropes, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.rmax,
-                       S.split_right, n);
+                       S.kleft, S.kright, S.kparent, S.split_right, S.krmin, S.krmax);
+    hipLaunchKernelGGL(k_preorder, dim3((nn + 255) / 256), dim3(256), 0, st, n, S.kleft, S.kparent,
+                       S.krmin, S.pre);
+    hipLaunchKernelGGL(k_permute, dim3((nn + 255) / 256), dim3(256), 0, st, n, S.pre, S.kright,
+                       S.kparent, S.split_right, S.krmin, S.krmax, S.sorted_ids, S.nodes, S.parent,
+                       S.nright, S.rmin, S.rmax, S.lpre, S.leaf_of);
     CS_HIP_RET(hipMemsetAsync(S.counters + CNT_TMP, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_mark_top, dim3((n - 1 + 255) / 256), dim3(256), 0, st, S.nodes, S.nright,
+    hipLaunchKernelGGL(k_mark_top, dim3((nn + 255) / 256), dim3(256), 0, st, S.nodes, S.nright,
                        S.rmin, S.rmax, n, S.top_start, S.counters + CNT_TMP);
     int cnt = 0;
     CS_HIP_RET(hipMemcpyAsync(&cnt, S.counters + CNT_TMP, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -779,12 +826,12 @@ hipError_t build_bvh(hipStream_t st, DevState& S) {
 hipError_t refit_bvh(hipStream_t st, DevState& S) {
   const int n = S.ne;
   if (n == 0) return hipSuccess;
-  if (n > 1) CS_HIP_RET(hipMemsetAsync(S.arrive, 0, sizeof(uint32_t) * (size_t)(n - 1), st));
+  if (n > 1) CS_HIP_RET(hipMemsetAsync(S.arrive, 0, sizeof(uint32_t) * (size_t)(2 * n - 1), st));
   hipLaunchKernelGGL(k_refit_tiles, dim3((n + REFIT_TILE - 1) / REFIT_TILE), dim3(REFIT_THREADS), 0,
-                     st, S.ebox, S.sorted_ids, n, S.nodes, S.parent, S.nright, S.split_right,
-                     S.rmin, S.rmax, S.arrive);
+                     st, S.ebox, S.sorted_ids, n, S.nodes, S.parent, S.nright, S.lpre, S.rmin,
+                     S.rmax, S.arrive);
   if (n > 1 && S.n_top_start > 0)
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, n, S.parent, S.nright,
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, S.nodes, S.parent, S.nright,
                        S.rmin, S.rmax, S.top_start, S.n_top_start, S.arrive);
   return hipGetLastError();
 }
@@ -793,7 +840,7 @@ void launch_traverse(hipStream_t st, DevState& S, int cap) {
   const int n = S.ne;
   if (n == 0) return;
   hipLaunchKernelGGL(k_traverse, dim3((n + TRAV_THREADS - 1) / TRAV_THREADS), dim3(TRAV_THREADS), 0,
-                     st, S.nodes, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
+                     st, S.nodes, S.lpre, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
                      (unsigned long long*)(S.counters + CNT_CAND64));
 }
 

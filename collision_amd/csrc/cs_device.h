@@ -25,15 +25,15 @@ struct __align__(16) dbox {  // FP64 element box, inflated by 1e-6
   double hi[3];
 };
 
-// LBVH node, one 32-byte record.  Internal nodes 0..n-2 and leaves
-// n-1..2n-2 (leaf j = Morton-sorted element j) share one array.
-//   internal: FP32 box of the subtree (rounded outward), left = first child,
+// LBVH node, one 32-byte record.  Nodes are stored in depth-first pre-order
+// (root first; an internal node's first child is the next record).
+//   internal: FP32 box of the subtree (rounded outward), left = -1,
 //             rope = next node after the subtree in depth-first order
 //   leaf:     FP32 box of the element (rounded outward), left = element id,
 //             rope as above
 // rope = -1 ends a traversal.  Depth-first traversal needs no stack: on an
-// overlap go to `left` (internal) or handle the leaf and go to `rope`; on a
-// miss go to `rope`.
+// overlap go to the next record (internal) or handle the leaf and go to
+// `rope`; on a miss go to `rope`.  Every step moves forward in memory.
 struct __align__(16) bvh_node {
   float lo[3];
   int32_t left;

@@ -95,12 +95,17 @@ struct DevState {
   uint32_t *sorted_codes = nullptr, *sorted_ids = nullptr;
   uint32_t* sort_hist = nullptr;
   size_t sort_hist_words = 0;
-  bvh_node *nodes = nullptr, *leaves = nullptr;
-  // nodes: 2n-1 records (internal 0..n-2, then leaves); leaves = nodes + n-1
-  int32_t* parent = nullptr;       // parent of every node (internal and leaf), -1 at the root
-  int32_t* nright = nullptr;       // second child of each internal node
+  bvh_node* nodes = nullptr;
+  // nodes: 2n-1 records in depth-first pre-order (root 0; an internal
+  // node's first child is the next record)
+  int32_t* parent = nullptr;       // parent of every node, -1 at the root
+  int32_t* nright = nullptr;       // second child of an internal node (-1 at leaves)
+  int32_t *rmin = nullptr, *rmax = nullptr;  // Morton leaf range of every node
+  int32_t* lpre = nullptr;         // node of Morton leaf j
+  // construction arrays (radix-tree order) of the last build
+  int32_t *kleft = nullptr, *kright = nullptr, *kparent = nullptr;
+  int32_t *krmin = nullptr, *krmax = nullptr, *pre = nullptr;
   int32_t* split_right = nullptr;  // [g]: second child of the node splitting after leaf g
-  int32_t *rmin = nullptr, *rmax = nullptr;
   uint32_t* arrive = nullptr;
   int32_t* top_start = nullptr;  // nodes spanning several refit tiles whose children do not
   int n_top_start = 0;           // (built once per LBVH build)
