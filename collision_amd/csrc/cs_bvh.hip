@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
   __shared__ unsigned long long s_cand[TRAV_THREADS / 64];
   __shared__ int s_base;
   const int t = threadIdx.x;
-  const int i = blockIdx.x * TRAV_THREADS + t;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * TRAV_THREADS + t;
   unsigned long long cand = 0;
   int nbuf = 0;
   if (i < n && n > 1) {
@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
   const int ndirty = counters[CNT_DIRTY];
   const int rounds = (ndirty + (int)(gridDim.x * TRAV_THREADS) - 1) / (int)(gridDim.x * TRAV_THREADS);
   for (int round = 0; round < rounds; ++round) {
-  const int k = (round * (int)gridDim.x + (int)blockIdx.x) * TRAV_THREADS + t;
+  const int k = (round * (int)gridDim.x + xcd_block(blockIdx.x, gridDim.x)) * TRAV_THREADS + t;
   unsigned long long cand = 0;
   int nbuf = 0;
   if (k < ndirty && n > 1) {

@@ -203,6 +203,18 @@ void launch_zone_labels(hipStream_t st, DevState& S, int32_t* out);
 // Work-group aggregated append: every lane of the block calls it; lanes with
 // want=true get consecutive slots from one global atomic per block.
 // s_cnt / s_base are __shared__ ints of the caller.
+// Work-groups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Remapping block b to a contiguous share per XCD gives every XCD one
+// contiguous range of the work (neighbouring leaves / pairs), so its L2
+// sees one region of the tree instead of a scattered 1/8 of all of it.
+// A bijection of [0, G).
+constexpr int kNumXcd = 8;
+__device__ inline int xcd_block(int b, int G) {
+  const int q = G / kNumXcd, r = G % kNumXcd;
+  const int x = b % kNumXcd, i = b / kNumXcd;
+  return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 __device__ inline int block_append(bool want, int* counter, int* s_cnt, int* s_base) {
   if (threadIdx.x == 0) *s_cnt = 0;
   __syncthreads();

@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(NARROW_THREADS, 3)
              ZoneList Z, hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp,
              int hp_cap, int* __restrict__ counters) {
   const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
-  const int64_t gl = (int64_t)blockIdx.x * NARROW_THREADS + threadIdx.x;
+  const int64_t gl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * NARROW_THREADS + threadIdx.x;
   if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
   narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
 }
