@@ -444,8 +444,30 @@ __device__ __attribute__((always_inline)) inline bool edge_over_limit_l(const do
                                          bool* degenerate) {
   const double xc0[3] = {o0[0] + dt * a0.x, o0[1] + dt * a0.y, o0[2] + dt * a0.z};
   const double xc1[3] = {o1[0] + dt * a1.x, o1[1] + dt * a1.y, o1[2] + dt * a1.z};
-  const double len_new = dist3(xc0, xc1);
+  double sq = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double d = xc0[i] - xc1[i];
+    sq += d * d;
+  }
   *degenerate = false;
+  // Fast decision, taken only far from every threshold: the length and the
+  // two quotients from the hardware reciprocal / reciprocal square root
+  // (accurate to far better than 1e-6 relative) differ from their correctly
+  // rounded values by much less than the 1e-4 margin, so the outcome equals
+  // the exact test's.  Inputs near a threshold, degenerate or non-finite
+  // ones take the exact path below.
+  if (sq > 0.0 && len_old > 2.0 * CS_ROUND_EPS) {
+    const double ln = sq * __builtin_amdgcn_rsq(sq);
+    if (ln > 2.0 * CS_ROUND_EPS) {
+      const double rate = (ln - len_old) * __builtin_amdgcn_rcp(len_old);
+      const double strn = (ln - len0) * __builtin_amdgcn_rcp(len0);
+      const double m = fmax(fabs(rate), fabs(strn));
+      if (m > CS_SUPERELAS_TOL + 1e-4) return true;
+      if (m < CS_SUPERELAS_TOL - 1e-4) return false;
+    }
+  }
+  const double len_new = sqrt(sq);  // dist3 of the same points
   if (len_old > CS_ROUND_EPS && len_new > CS_ROUND_EPS) {
     const double strain_rate = (len_new - len_old) / len_old;
     const double strain = (len_new - len0) / len0;
