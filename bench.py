@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=256, help="cloth side of the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run three 256 MiB device copies (scale for HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """Three plain device copies of 256 MiB: the reference for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the collision kernels to bytes."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    x = torch.arange(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y = x.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copies": 3, "last": float(y[-1])}))
 
 
 def cpu_baseline(args, ne_full):
@@ -77,6 +92,9 @@ def cpu_baseline(args, ne_full):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the collision step "
               f"of one scene is not sharded across GPUs in this build.", file=sys.stderr)
