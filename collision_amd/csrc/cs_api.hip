@@ -104,7 +104,7 @@ void free_dev(DevState& S) {
   dfree(S.zkey); dfree(S.zval); dfree(S.zval_tmp);
   dfree(S.zhead); dfree(S.zend);
   dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters);
-  dfree(S.shead); dfree(S.snl); dfree(S.sbody); dfree(S.sflag);
+  dfree(S.sslot); dfree(S.spre); dfree(S.slast);
   dfree(S.lvl_active);
   S = DevState();
 }
@@ -561,71 +561,70 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   for (int v = 0; v < nv; ++v)
     if (!used[v])
       return fail(h, CS_ERR_ARG, "every vertex must belong to a triangle or bond");
-  // strain-limit levels: 1 + max level of earlier non-rigid elements that
-  // share a vertex (dcollid.cpp:492-557 visits elements in order)
-  std::vector<int> last((size_t)nv, -1), level((size_t)std::max(ne, 1), -1);
-  int nlev = 0, nsched = 0;
-  for (int e = 0; e < ne; ++e) {
-    if (ef[e] & EF_RIGID) continue;
-    const int np = (ef[e] & EF_TRI) ? 3 : 2;
-    const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
-    int l = 0;
-    for (int i = 0; i < np; ++i) l = std::max(l, last[vv[i]] + 1);
-    for (int i = 0; i < np; ++i) last[vv[i]] = l;
-    level[e] = l;
-    nlev = std::max(nlev, l + 1);
-    ++nsched;
-  }
-  // slots: the scheduled elements in level order; within a level, element
-  // order (any order is exact there: the elements of one level share no
-  // vertex)
+  // strain-limit schedule: levels (cs_schedule.hip), then slots in level
+  // order, element order within a level (the elements of one level share no
+  // vertex, so any order is exact there)
+  std::vector<int> level;
+  int nlev = 0, sdelta = 1;
+  build_strain_schedule(nv, ne, ev.data(), ef.data(), level, nlev, sdelta);
+  int nsched = 0;
+  for (int e = 0; e < ne; ++e)
+    if (level[e] >= 0) ++nsched;
   std::vector<int> lptr((size_t)nlev + 1, 0), slot_of((size_t)std::max(ne, 1), -1);
   const size_t nslot = (size_t)std::max(nsched, 1);
   for (int e = 0; e < ne; ++e)
     if (level[e] >= 0) lptr[level[e] + 1]++;
-  for (int l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+  for (int l = 0; l < nlev; ++l) {
+    if (lptr[l + 1] > STRAIN_LEVEL_CAP)
+      return fail(h, CS_ERR_INTERNAL, "strain schedule level above its size cap");
+    lptr[l + 1] += lptr[l];
+  }
   {
     std::vector<int> fill(lptr.begin(), lptr.end() - 1);
     for (int e = 0; e < ne; ++e)
       if (level[e] >= 0) slot_of[e] = fill[level[e]]++;
   }
-  std::vector<StrainHead> shead(nslot);
-  std::vector<int4> snl(nslot, make_int4(-1, -1, -1, -1));
-  std::vector<StrainBody> sbody(nslot);
-  std::memset(shead.data(), 0xff, sizeof(StrainHead) * nslot);
-  std::memset(sbody.data(), 0, sizeof(StrainBody) * nslot);
+  std::vector<StrainSlot> sslot(nslot);
+  std::vector<int> slast;
+  std::memset(sslot.data(), 0, sizeof(StrainSlot) * nslot);
   {
-    // the next visit of each vertex: the next scheduled element holding it
-    // (element order and level order agree on the elements of one vertex)
-    std::vector<int> nxt_slot((size_t)nv, -1), nxt_pos((size_t)nv, -1);
-    for (int e = ne - 1; e >= 0; --e) {
-      if (level[e] < 0) continue;
-      const bool tri = (ef[e] & EF_TRI) != 0;
-      const int np = tri ? 3 : 2;
-      const int vv[3] = {ev[e].x, ev[e].y, tri ? ev[e].z : -1};
-      const int s = slot_of[e];
-      StrainHead& hd = shead[s];
-      int nl[3] = {-1, -1, -1};
+    // links: each visit of a vertex to its next visit, the last to the
+    // first (element order and level order agree on the elements of one
+    // vertex)
+    std::vector<int> first((size_t)nv, -1), last((size_t)nv, -1);
+    for (size_t s = 0; s < nslot; ++s)
       for (int i = 0; i < 3; ++i) {
-        hd.v[i] = vv[i];
-        hd.ns[i] = -1;
+        sslot[s].v[i] = -1;
+        sslot[s].ns[i] = -1;
+        sslot[s].nl[i] = -1;
       }
+    for (int e = 0; e < ne; ++e) {
+      if (level[e] < 0) continue;
+      const int np = (ef[e] & EF_TRI) ? 3 : 2;
+      const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
+      const int s = slot_of[e];
+      StrainSlot& r = sslot[s];
+      r.level = level[e];
+      r.L0[0] = l0[e].x; r.L0[1] = l0[e].y; r.L0[2] = l0[e].z;
       for (int i = 0; i < np; ++i) {
-        const int ns = nxt_slot[vv[i]];
-        if (ns >= 0) {
-          hd.ns[i] = 4 * ns + nxt_pos[vv[i]];  // byte of the next slot's flag word
-          nl[i] = shead[ns].level;
+        const int v = vv[i];
+        r.v[i] = v;
+        if (last[v] >= 0) {
+          StrainSlot& p = sslot[last[v] >> 2];
+          p.ns[last[v] & 3] = 4 * s + i;
+          p.nl[last[v] & 3] = level[e];
+        } else {
+          first[v] = 4 * s + i;
         }
+        last[v] = 4 * s + i;
       }
-      snl[s] = make_int4(nl[0], nl[1], nl[2], -1);
-      hd.level = level[e];
-      hd.elem = e;
-      for (int i = 0; i < np; ++i) {
-        nxt_slot[vv[i]] = s;
-        nxt_pos[vv[i]] = i;
-      }
-      StrainBody& bd = sbody[s];
-      bd.L0[0] = l0[e].x; bd.L0[1] = l0[e].y; bd.L0[2] = l0[e].z;
+    }
+    slast = last;
+    for (int v = 0; v < nv; ++v) {
+      if (last[v] < 0) continue;
+      StrainSlot& p = sslot[last[v] >> 2];
+      p.ns[last[v] & 3] = first[v];
+      p.nl[last[v] & 3] = sslot[first[v] >> 2].level;
     }
   }
 
@@ -673,9 +672,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   S.n_levels = nlev;
   S.n_sched = nsched;
   HIPCHK(h, dalloc(&S.lvl_ptr, nlev + 1));
-  HIPCHK(h, dalloc(&S.shead, nslot)); HIPCHK(h, dalloc(&S.snl, nslot));
-  HIPCHK(h, dalloc(&S.sbody, nslot));
-  HIPCHK(h, dalloc(&S.sflag, nslot)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
+  S.strain_delta = sdelta;
+  HIPCHK(h, dalloc(&S.sslot, nslot));
+  HIPCHK(h, dalloc(&S.spre, nslot)); HIPCHK(h, dalloc(&S.slast, nv)); HIPCHK(h, dalloc(&S.lvl_active, std::max(nlev, 1)));
   S.plog_cap = 1 << 20;
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
   HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
@@ -685,10 +684,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.len0, l0.data(), sizeof(dvec4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.fixed, fixed.data(), nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.lvl_ptr, lptr.data(), sizeof(int) * (nlev + 1), hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(S.shead, shead.data(), sizeof(StrainHead) * nslot, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(S.snl, snl.data(), sizeof(int4) * nslot, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemcpy(S.sbody, sbody.data(), sizeof(StrainBody) * nslot, hipMemcpyHostToDevice));
-  HIPCHK(h, hipMemset(S.sflag, 0, sizeof(uint32_t) * nslot));
+  HIPCHK(h, hipMemcpy(S.sslot, sslot.data(), sizeof(StrainSlot) * nslot, hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemset(S.spre, 0, sizeof(uint32_t) * nslot));
+  HIPCHK(h, hipMemcpy(S.slast, slast.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
   HIPCHK(h, hipMemset(S.has, 0, nv));
   HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
@@ -803,7 +801,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
     launch_final_position(h->st, S, P.dt);
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_FLAG, 0, sizeof(int) * 4, h->st));
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_LEVELS, 0, sizeof(int) * 2, h->st));
-    HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps, nullptr));
+    HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps));
     launch_final_velocity(h->st, S);
     HIPCHK(h, hipGetLastError());
   }

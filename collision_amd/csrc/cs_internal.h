@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "cs_device.h"
 
 #define CS_HIP_RET(expr)                    \
@@ -56,26 +58,26 @@ struct NarrowParams {
   int pass_id;
 };
 
-// One slot of the strain-limit schedule (one non-rigid element).  The head
-// (32 B) is what a visit of the slot reads first; the body (128 B) is read
-// only when the slot is evaluated; the next-visit levels only when a dirty
-// vertex's next visit is not on the following level.  Rest lengths are
-// written at assembly; x_old of the vertices and the x_old edge lengths are
-// packed once per step.
-struct alignas(16) StrainHead {
-  int v[3];    // vertex ids; v[2] = -1 for a bond
-  int ns[3];   // byte index (4 * slot + position) of the next visit's dirty flag (-1: none)
-  int level;
-  int elem;    // element id
-};
-static_assert(sizeof(StrainHead) == 32, "StrainHead layout");
-struct alignas(16) StrainBody {
+// One slot of the strain-limit schedule (one non-rigid element), 256 B.
+// Rest lengths, vertex ids and links are written at assembly; x_old of the
+// vertices, the x_old edge lengths and the velocity copies are packed once
+// per step.  Each slot keeps its own copy of its vertices' velocities: a
+// visit that changes a vertex (or passes a change on) stores the new value
+// into the copy held by the vertex's next visit, so an evaluated slot reads
+// everything it needs from its own record.
+struct alignas(16) StrainSlot {
   double x[9];     // x_old of vertices 0, 1, 2 (vertex 2 unused for a bond)
   double lold[3];  // x_old lengths of edges (0,1), (1,2), (2,0)
   double L0[3];    // rest lengths of the same edges
-  double unused;
+  int v[3];        // vertex ids; v[2] = -1 for a bond
+  int ns[3];       // 4 * slot + position of the vertex's next visit; the last
+                   // visit links to the first (a single visit to itself); -1: none
+  int nl[3];       // level of that visit
+  int level;
+  dvec4 vel[3];    // velocity copies; w holds the bits of the slot that last wrote the vertex
 };
-static_assert(sizeof(StrainBody) == 128, "StrainBody layout");
+static_assert(sizeof(StrainSlot) == 256, "StrainSlot layout");
+constexpr int STRAIN_LEVEL_CAP = 1024;  // most slots in one level
 
 // All device memory of one solver handle.
 struct DevState {
@@ -145,12 +147,12 @@ struct DevState {
   // strain-limit schedule: slots in level order, slots of level L are
   // [lvl_ptr[L], lvl_ptr[L+1])
   int32_t* lvl_ptr = nullptr;
-  StrainHead* shead = nullptr;
-  int4* snl = nullptr;             // per slot: level of each vertex's next visit (-1: none)
-  StrainBody* sbody = nullptr;
-  uint32_t* sflag = nullptr;       // per slot: bytes 0-2 vertex dirty, byte 3 pre-check active
+  StrainSlot* sslot = nullptr;
+  uint32_t* spre = nullptr;        // per slot: byte 3 pre-check active, bytes 0-2 far marks
+  int32_t* slast = nullptr;        // per vertex: 4 * slot + position of its last visit (-1: none)
   uint8_t* lvl_active = nullptr;   // level holds an active element (pre-check)
   int n_levels = 0, n_sched = 0;
+  int strain_delta = 1;            // 1 + widest level span of a vertex's visits
   // logs
   int4* plog = nullptr;
   int plog_cap = 0;
@@ -196,7 +198,10 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nl
 void launch_domain_boundary(hipStream_t st, DevState& S, const double lo[3], const double hi[3],
                             double dt, double mu, double thickness);
 void launch_final_position(hipStream_t st, DevState& S, double dt);
-hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, int* sweeps_out);
+hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps);
+// cs_schedule.hip (host)
+void build_strain_schedule(int nv, int ne, const int4* ev, const uint8_t* ef,
+                           std::vector<int>& level, int& n_levels, int& delta);
 void launch_final_velocity(hipStream_t st, DevState& S);
 void launch_zone_labels(hipStream_t st, DevState& S, int32_t* out);
 

@@ -477,81 +477,97 @@ __device__ __attribute__((always_inline)) inline bool edge_over_limit_l(const do
   return true;
 }
 
-// Sparse exact sweep.  An element none of whose vertices was written earlier
-// in the sweep reads the pre-sweep velocities, so whether it writes is
-// decided by the parallel pre-check.  Only pre-check-active elements and
-// later visits of written ("dirty") vertices are evaluated, in level order.
+// Pipelined sparse exact sweeps.
 //
-// Dirty state travels along each vertex's chain of visits: a slot that
-// evaluates a dirty vertex marks that vertex in the slot of its next visit,
-// so every later element holding the vertex is evaluated, as in a
-// sequential sweep.  Almost every next visit is on the following level; such
-// marks go to an LDS table for that level, the rest to the slot's flag word
-// in memory (bytes 0-2 dirty vertices, byte 3 pre-check active) with an LDS
-// bit telling that level to re-read its flag words.  The heads and flag
-// words of the following level and of the next pre-check-active level are
-// loaded one level ahead, so a level's critical path is: LDS marks -> body and
-// velocities of the evaluated slots (one memory round trip) -> arithmetic ->
-// barrier.  Levels with nothing to do are skipped using LDS level bits.
+// Sweeps run in one work-group, level by level; at iteration t sweep k works
+// on level t - k * delta, where delta exceeds the level span of every
+// vertex's visits.  A vertex's visits in sweep k + 1 therefore come after
+// all its visits in sweep k, and every element sees the same velocities as
+// in the sequential sweep loop.  Within one iteration the levels being
+// processed share no vertex.
+//
+// A slot (element) is evaluated in sweep k only when its result can differ
+// from "nothing over the limit":
+//   * it wrote in sweep k - 1 (for the first sweep of a launch: the parallel
+//     pre-check, on the velocities at the start of the sweep, found an edge
+//     over the limit), or
+//   * one of its vertices changed since its evaluation in sweep k - 1.
+// Otherwise its inputs are the ones it saw, unchanged, in sweep k - 1, where
+// nothing was over the limit.  Changes travel along each vertex's cycle of
+// visits as a token: the visit that writes a vertex stores the new value
+// into the next visit's velocity copy and marks it; a marked visit that does
+// not write passes the value on unless the next visit is the writer itself
+// (whose copy already holds the value).  A sweep in which nothing is over the
+// limit ends the loop; it leaves no token and no written slot behind, so the
+// sweeps after it evaluate nothing.
+//
+// Marks of the levels that may hold pending work (at most 64, as
+// K * delta <= 64) live in an LDS ring, one byte per slot: bits 0-2 vertex
+// changed, bit 3 wrote in the previous sweep.  With a single sweep in flight
+// a mark 64 or more levels ahead goes to the slot's word in global memory.
 
-// x_old of the slot vertices and edge lengths, once per step
-__global__ void k_strain_pack(const StrainHead* __restrict__ head, StrainBody* __restrict__ body,
-                              const dvec4* __restrict__ x_old, int n) {
+// x_old of the slot vertices, edge lengths and velocity copies
+__global__ void k_strain_pack(StrainSlot* __restrict__ slots, const dvec4* __restrict__ x_old,
+                              const dvec4* __restrict__ av, int n, int with_x) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const StrainHead h = head[s];
-  StrainBody& r = body[s];
-  const dvec4 a = x_old[h.v[0]], b = x_old[h.v[1]];
-  const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
-  r.x[0] = a.x; r.x[1] = a.y; r.x[2] = a.z;
-  r.x[3] = b.x; r.x[4] = b.y; r.x[5] = b.z;
-  r.lold[0] = dist3(pa, pb);
-  if (h.v[2] >= 0) {
-    const dvec4 c = x_old[h.v[2]];
-    const double pc[3] = {c.x, c.y, c.z};
-    r.x[6] = c.x; r.x[7] = c.y; r.x[8] = c.z;
-    r.lold[1] = dist3(pb, pc);
-    r.lold[2] = dist3(pc, pa);
+  StrainSlot& r = slots[s];
+  const int v0 = r.v[0], v1 = r.v[1], v2 = r.v[2];
+  const double none = __longlong_as_double(-1ll);
+  if (with_x) {
+    const dvec4 a = x_old[v0], b = x_old[v1];
+    const double pa[3] = {a.x, a.y, a.z}, pb[3] = {b.x, b.y, b.z};
+    r.x[0] = a.x; r.x[1] = a.y; r.x[2] = a.z;
+    r.x[3] = b.x; r.x[4] = b.y; r.x[5] = b.z;
+    r.lold[0] = dist3(pa, pb);
+    if (v2 >= 0) {
+      const dvec4 c = x_old[v2];
+      const double pc[3] = {c.x, c.y, c.z};
+      r.x[6] = c.x; r.x[7] = c.y; r.x[8] = c.z;
+      r.lold[1] = dist3(pb, pc);
+      r.lold[2] = dist3(pc, pa);
+    }
+  }
+  const dvec4 a0 = av[v0], a1 = av[v1];
+  r.vel[0] = dvec4{a0.x, a0.y, a0.z, none};
+  r.vel[1] = dvec4{a1.x, a1.y, a1.z, none};
+  if (v2 >= 0) {
+    const dvec4 a2 = av[v2];
+    r.vel[2] = dvec4{a2.x, a2.y, a2.z, none};
   }
 }
 
-// pre-check with the current velocities: flag words (dirty bytes cleared,
-// active byte set), lvl_active and the any-edge flag
-__global__ void k_strain_check(const StrainHead* __restrict__ head,
-                               const StrainBody* __restrict__ body, const dvec4* __restrict__ av,
-                               double dt, int n, uint32_t* __restrict__ sflag,
-                               uint8_t* __restrict__ lvl_active, int* __restrict__ counters) {
+// pre-check with the velocities at the start of the sweep: the slot's word
+// (byte 3 set when an edge is over the limit, far marks cleared), lvl_active
+// and the any-edge flag
+__global__ void k_strain_check(const StrainSlot* __restrict__ slots, double dt, int n,
+                               uint32_t* __restrict__ spre, uint8_t* __restrict__ lvl_active,
+                               int* __restrict__ counters) {
   if (counters[CNT_STRAIN_FLAG] < 0) return;  // loop already finished
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   bool any = false;
   if (s < n) {
-    const StrainHead h = head[s];
-    const StrainBody& r = body[s];
-    const dvec4 a0 = av[h.v[0]], a1 = av[h.v[1]];
+    const StrainSlot& r = slots[s];
+    const dvec4 a0 = r.vel[0], a1 = r.vel[1];
     bool deg;
     if (edge_over_limit_l(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, &deg)) any = true;
-    if (h.v[2] >= 0) {
-      const dvec4 a2 = av[h.v[2]];
+    if (r.v[2] >= 0) {
+      const dvec4 a2 = r.vel[2];
       if (edge_over_limit_l(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, &deg)) any = true;
       if (edge_over_limit_l(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, &deg)) any = true;
     }
-    sflag[s] = any ? 0x01000000u : 0u;
-    if (any) lvl_active[h.level] = 1;
+    spre[s] = any ? 0x01000000u : 0u;
+    if (any) lvl_active[r.level] = 1;
   }
   if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(&counters[CNT_STRAIN_FLAG], 1);
 }
 
-constexpr int STRAIN_THREADS = 1024;
-constexpr int STRAIN_TAB = STRAIN_THREADS;  // slots of a level on the LDS-mark path
-constexpr int STRAIN_LDS_LEVELS = 65536;    // levels whose bits fit in LDS
-constexpr int STRAIN_LW = STRAIN_LDS_LEVELS / 32;
-constexpr int STRAIN_LP_MAX = 8192;         // levels whose slot offsets are kept in LDS
-
-struct StrainCounts {
-  int* flag;
-  int* edges;
-  int* degen;
-};
+constexpr int SP_THREADS = 512;
+constexpr int SP_RING = 64;            // levels of marks held in LDS
+constexpr int SP_KMAX = 10;            // sweeps in flight
+constexpr int SP_LP_MAX = 8192;        // levels whose slot offsets are kept in LDS
+constexpr int SP_PRE_LEVELS = 65536;   // levels whose pre-check bits are kept in LDS
+static_assert(STRAIN_LEVEL_CAP % 4 == 0 && STRAIN_LEVEL_CAP <= 1024, "work entry layout");
 
 __device__ inline bool bit_get(const uint32_t* b, int i) { return (b[i >> 5] >> (i & 31)) & 1u; }
 __device__ inline void bit_set(uint32_t* b, int i) { atomicOr(&b[i >> 5], 1u << (i & 31)); }
@@ -566,281 +582,342 @@ __device__ inline int bit_next(const uint32_t* b, int from, int n) {
   return n;
 }
 
-// average the two velocities of an edge found over the limit
-__device__ inline void strain_average(dvec4& a, dvec4& b, bool deg, int ia, int ib, dvec4* av,
-                                      const StrainCounts& C) {
-  double vt[3] = {a.x + b.x, a.y + b.y, a.z + b.z};
-  vt[0] = 0.5 * vt[0];
-  vt[1] = 0.5 * vt[1];
-  vt[2] = 0.5 * vt[2];
-  const dvec4 nv4 = dvec4{vt[0], vt[1], vt[2], 0.0};
-  a = nv4;
-  b = nv4;
-  av[ia] = nv4;
-  av[ib] = nv4;
-  *C.flag = 1;
-  atomicAdd(C.edges, 1);
-  if (deg) atomicAdd(C.degen, 1);
+// velocity copy <-> (value, writer slot)
+__device__ inline dvec4 vel_with_origin(const dvec4& a, int origin) {
+  return dvec4{a.x, a.y, a.z, __longlong_as_double((long long)origin)};
+}
+__device__ inline int origin_of(const dvec4& a) { return (int)__double_as_longlong(a.w); }
+
+// Every vertex's current velocity is in the copy of its last visit: a
+// write's token reaches the last visit within the sweep of the write, and
+// nothing changes the vertex after its last write.
+__global__ void k_strain_writeback(const StrainSlot* __restrict__ slots,
+                                   const int32_t* __restrict__ slast, dvec4* __restrict__ av,
+                                   int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int lv = slast[v];
+  if (lv < 0) return;
+  const dvec4 c = slots[lv >> 2].vel[lv & 3];
+  av[v] = dvec4{c.x, c.y, c.z, 0.0};
 }
 
-// where a dirty vertex's next visit is marked
-struct StrainMarks {
-  uint32_t* tab_next;   // LDS marks of level L + 1 (fast path), indexed from nb
-  int nb, ne;           // slot range of level L + 1; ne == nb when L + 1 is not on the fast path
-  uint8_t* sflag_b;     // flag bytes in memory (other levels)
-  const int4* snl;      // next-visit levels (for marks in memory)
-  uint32_t* actb;       // LDS: level has work
-  uint32_t* gscb;       // LDS: level must re-read its flag words
-  bool use_lds;
-};
-
-// evaluate one slot's edges in order, (0,1), (1,2), (2,0), with the current
-// velocities (reduceSuperelast's per-element loop, dcollid.cpp:515-560)
-__device__ __attribute__((always_inline)) inline void strain_eval(const StrainHead& h, uint32_t f, const StrainBody* body,
-                                   int s, int next_level, dvec4* av, const StrainMarks M,
-                                   double dt, const StrainCounts C) {
-  bool d0 = (f & 0xffu) != 0, d1 = (f & 0xff00u) != 0, d2 = (f & 0xff0000u) != 0;
-  const bool tri = h.v[2] >= 0;
-  // the whole body and all velocities are loaded before the first store
-  // (the stores to av would otherwise order later body loads behind them)
-  const StrainBody r = body[s];
-  dvec4 a0 = av[h.v[0]], a1 = av[h.v[1]], a2{};
-  if (tri) a2 = av[h.v[2]];
-  bool g;
-  if (edge_over_limit_l(r.x, r.x + 3, a0, a1, r.lold[0], r.L0[0], dt, &g)) {
-    strain_average(a0, a1, g, h.v[0], h.v[1], av, C);
-    d0 = d1 = true;
-  }
-  if (tri) {
-    if (edge_over_limit_l(r.x + 3, r.x + 6, a1, a2, r.lold[1], r.L0[1], dt, &g)) {
-      strain_average(a1, a2, g, h.v[1], h.v[2], av, C);
-      d1 = d2 = true;
-    }
-    if (edge_over_limit_l(r.x + 6, r.x, a2, a0, r.lold[2], r.L0[2], dt, &g)) {
-      strain_average(a2, a0, g, h.v[2], h.v[0], av, C);
-      d2 = d0 = true;
-    }
-  }
-  // every later visit of a dirty vertex must be evaluated
-  const bool dk[3] = {d0, d1, d2};
+// sum over the wavefront
+__device__ inline int wave_sum(int x) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int ns = h.ns[k];
-    if (!dk[k] || ns < 0) continue;
-    const int slot = ns >> 2;
-    if (slot >= M.nb && slot < M.ne) {  // next visit on the following level
-      atomicOr(&M.tab_next[slot - M.nb], 1u << (8 * (ns & 3)));
-      if (M.use_lds) bit_set(M.actb, next_level);
-    } else {
-      M.sflag_b[ns] = 1;
-      if (M.use_lds) {
-        const int4 nl4 = M.snl[s];
-        const int nl = k == 0 ? nl4.x : (k == 1 ? nl4.y : nl4.z);
-        bit_set(M.actb, nl);
-        bit_set(M.gscb, nl);
-      }
-    }
-  }
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
 }
 
-// a head held in registers as two int4 (v0 v1 v2 ns0 | ns1 ns2 level elem),
-// so a prefetched head never needs an addressable copy
-struct HeadRegs {
-  int4 a, b;
-};
-__device__ inline HeadRegs load_head(const StrainHead* __restrict__ head, int s) {
-  const int4* p = reinterpret_cast<const int4*>(head + s);
-  return HeadRegs{p[0], p[1]};
-}
-__device__ inline void store_head(StrainHead& h, const HeadRegs& r) {
-  h.v[0] = r.a.x; h.v[1] = r.a.y; h.v[2] = r.a.z;
-  h.ns[0] = r.a.w; h.ns[1] = r.b.x; h.ns[2] = r.b.y;
-  h.level = r.b.z; h.elem = r.b.w;
-}
-
-// One work-group walks the levels in order.  On the fast path (levels of at
-// most STRAIN_TAB slots) lane t owns slot b + t; larger levels are walked
-// with a stride and always read their flag words from memory.
+// Work entry of one iteration: bits 0-9 slot index within its level, bits
+// 10-13 sweep in flight, bits 14-17 marks.
 template <bool LP_LDS>
-__global__ void __launch_bounds__(STRAIN_THREADS)
-    k_strain_sweep(const int32_t* __restrict__ lvl_ptr, int n_levels,
-                   const StrainHead* __restrict__ head, const int4* __restrict__ snl,
-                   const StrainBody* __restrict__ body, uint32_t* sflag,
-                   const uint8_t* __restrict__ lvl_active_g, dvec4* av, double dt,
-                   int* __restrict__ counters) {
-  __shared__ uint32_t actb[STRAIN_LW], preb[STRAIN_LW], gscb[STRAIN_LW];
-  __shared__ uint32_t tab[2][STRAIN_TAB];
-  __shared__ int32_t lp[STRAIN_LP_MAX + 1];
-  // slots of the current level to evaluate, packed so that the FP64 work
-  // of a level lands in as few wavefronts as possible
-  __shared__ StrainHead wk_head[STRAIN_TAB];
-  __shared__ int32_t wk_slot[STRAIN_TAB];
-  __shared__ uint32_t wk_f[STRAIN_TAB];
-  __shared__ int s_nwork[2];
-  __shared__ int s_flag, s_edges, s_degen, s_state, s_levels, s_evals;
-  const int t = threadIdx.x;
-  if (t == 0) {
-    s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweep needed
-    s_flag = 0;
-    s_edges = 0;
-    s_degen = 0;
-    s_levels = 0;
-    s_evals = 0;
-    s_nwork[0] = 0;
-    s_nwork[1] = 0;
+__global__ void __launch_bounds__(SP_THREADS)
+    k_strain_pipe(const int32_t* __restrict__ lvl_ptr, int n_levels, StrainSlot* slots,
+                  uint32_t* spre, const uint8_t* __restrict__ lvl_active_g, double dt, int K,
+                  int delta, int* __restrict__ counters) {
+  static_assert(STRAIN_LEVEL_CAP % SP_THREADS == 0, "whole slots of a level per lane");
+  constexpr int SPL = STRAIN_LEVEL_CAP / SP_THREADS;  // slots of a level per lane
+  __shared__ uint32_t ring[SP_RING][STRAIN_LEVEL_CAP / 4];
+  __shared__ uint32_t wk[SP_KMAX * STRAIN_LEVEL_CAP];
+  __shared__ int32_t lp[LP_LDS ? SP_LP_MAX + 1 : 1];
+  __shared__ uint32_t preb[SP_PRE_LEVELS / 32];
+  __shared__ int s_nwork[2], s_last_mark, s_state;
+  __shared__ uint32_t s_any[SP_THREADS / 64];
+  __shared__ int s_last_edges[SP_THREADS / 64], s_degen[SP_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) {
+    s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweeps needed
+    s_nwork[0] = s_nwork[1] = 0;
+    s_last_mark = INT32_MIN / 2;
   }
   __syncthreads();
   const int state = s_state;
   if (state < 0) return;
   if (state == 0) {  // nothing over the limit: this sweep is a no-op and ends the loop
-    if (t == 0) {
+    if (tid == 0) {
       counters[CNT_STRAIN_FLAG] = -1;
       counters[CNT_STRAIN_EDGES] = 0;
       atomicAdd(&counters[CNT_STRAIN_SWEEPS], 1);
     }
     return;
   }
-  const StrainCounts C{&s_flag, &s_edges, &s_degen};
-  const bool use_lds = n_levels <= STRAIN_LDS_LEVELS;
-  const int nw = use_lds ? (n_levels + 31) / 32 : 0;
-  for (int w = t; w < nw; w += STRAIN_THREADS) {
-    uint32_t bits = 0;
-    for (int j = 0; j < 32; ++j) {
-      const int L = 32 * w + j;
-      if (L < n_levels && lvl_active_g[L]) bits |= 1u << j;
+  const bool use_preb = n_levels <= SP_PRE_LEVELS;
+  for (int i = tid; i < SP_RING * STRAIN_LEVEL_CAP / 4; i += SP_THREADS) (&ring[0][0])[i] = 0u;
+  if (use_preb)
+    for (int w = tid; w < (n_levels + 31) / 32; w += SP_THREADS) {
+      uint32_t bits = 0;
+      for (int j = 0; j < 32; ++j) {
+        const int L = 32 * w + j;
+        if (L < n_levels && lvl_active_g[L]) bits |= 1u << j;
+      }
+      preb[w] = bits;
     }
-    actb[w] = bits;
-    preb[w] = bits;
-    gscb[w] = 0u;
-  }
-  for (int i = t; i < 2 * STRAIN_TAB; i += STRAIN_THREADS) (&tab[0][0])[i] = 0u;
-  if (LP_LDS)
-    for (int i = t; i <= n_levels; i += STRAIN_THREADS) lp[i] = lvl_ptr[i];
+  if constexpr (LP_LDS)
+    for (int i = tid; i <= n_levels; i += SP_THREADS) lp[i] = lvl_ptr[i];
   __syncthreads();
-  // slot offset of level i (from LDS when the offsets fit there)
   auto lvp = [&](int i) -> int {
     if constexpr (LP_LDS) return lp[i];
     else return lvl_ptr[i];
   };
-  uint8_t* sflag_b = reinterpret_cast<uint8_t*>(sflag);
-  // prefetched head + flag word of this lane's slot: level L + 1 (pn) and
-  // the next pre-check-active level (pq)
-  HeadRegs pn{}, pq{};
-  uint32_t pnf = 0u, pqf = 0u;
-  int pn_level = -1, pq_level = -1;
-  int cur = 0, evals = 0;
-  int L = use_lds ? bit_next(actb, 0, n_levels) : 0;
-  while (L < n_levels) {
-    const int b = lvp(L), e = lvp(L + 1);
-    const bool fast = use_lds && (e - b) <= STRAIN_TAB;
-    // slots of the following level (mark target of the fast path)
-    int nb = 0, ne = 0, ne2 = 0;
-    if (L + 1 < n_levels) {
-      nb = e;
-      ne2 = lvp(L + 2);
-      ne = (use_lds && ne2 - nb <= STRAIN_TAB) ? ne2 : nb;
+  uint8_t* ring_b = reinterpret_cast<uint8_t*>(&ring[0][0]);
+  uint8_t* spre_b = reinterpret_cast<uint8_t*>(spre);
+  // The first sweep's pre-check results (and far marks) enter the ring one
+  // level ahead: the word of this lane's slot in level l is loaded with the
+  // slot records of an evaluation round, so waiting for it never waits for
+  // stores (the vector memory counter retires loads and stores in order),
+  // and the scan reads LDS only.
+  struct PreWords {
+    uint32_t w[SPL];
+  };
+  auto pre_word = [&](int l) -> PreWords {
+    PreWords p;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) p.w[j] = 0u;
+    if (l >= n_levels || (use_preb && !bit_get(preb, l))) return p;
+    const int b = lvp(l), n = lvp(l + 1) - b;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const int i = tid + j * SP_THREADS;
+      if (i < n) p.w[j] = spre[b + i];
     }
-    const StrainMarks M{tab[cur ^ 1], nb, ne, sflag_b, snl, actb, gscb, use_lds};
-    if (fast) {
-      HeadRegs h{};
-      uint32_t f = 0u;
-      const bool ok = b + t < e;
-      if (pn_level == L) {
-        h = pn;
-        f = pnf;
-      } else if (pq_level == L) {
-        h = pq;
-        f = pqf;
-      } else if (ok) {
-        h = load_head(head, b + t);
-        f = sflag[b + t];
+    return p;
+  };
+  auto fold_pre = [&](int l, const PreWords& p) {
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+      const uint32_t w = p.w[j];
+      if (!w) continue;
+      const int i = tid + j * SP_THREADS;
+      const uint32_t f = ((w >> 24) ? 8u : 0u) | (w & 1u) | ((w >> 7) & 2u) | ((w >> 14) & 4u);
+      atomicOr(&ring[l & (SP_RING - 1)][i >> 2], f << (8 * (i & 3)));
+      if (w & 0x00ffffffu) spre[lvp(l) + i] = w & 0xff000000u;  // far marks consumed
+    }
+  };
+  // per lane: sweeps with an edge over the limit (bit k), edges over the
+  // limit in the last sweep in flight, degenerate edges
+  uint32_t e_any = 0u;
+  int e_last = 0, d_acc = 0;
+  const int tend = n_levels + (K - 1) * delta;
+  int t = use_preb ? bit_next(preb, 0, n_levels) : 0;
+  int cur = 0, iters = 0;
+  long long evals = 0;
+  if (t < n_levels) fold_pre(t, pre_word(t));
+  __syncthreads();
+  while (t < tend) {
+    // --- scan: lane tid looks at slot tid of every level in flight ---
+    // (all offsets first, then all marks: two LDS round trips)
+    int lb[SP_KMAX], ln[SP_KMAX];
+#pragma unroll
+    for (int k = 0; k < SP_KMAX; ++k) {
+      const int l = t - k * delta;
+      const bool ok = k < K && l >= 0 && l < n_levels;
+      const int b = ok ? lvp(l) : 0, e = ok ? lvp(l + 1) : 0;
+      lb[k] = b;
+      ln[k] = e - b;
+    }
+    uint32_t fk[SP_KMAX][SPL];
+#pragma unroll
+    for (int k = 0; k < SP_KMAX; ++k)
+#pragma unroll
+      for (int j = 0; j < SPL; ++j) {
+        const int l = t - k * delta, i = tid + j * SP_THREADS;
+        fk[k][j] = i < ln[k] ? ring_b[(size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP + i] : 0u;
       }
-      if (ok && bit_get(gscb, L)) f = sflag[b + t];  // marks written to memory since the prefetch
-      const uint32_t m = tab[cur][t];
-      f |= m;
-      if (m) tab[cur][t] = 0u;
-      if (ok && f != 0u) {
-        const int k = atomicAdd(&s_nwork[cur], 1);
-        store_head(wk_head[k], h);
-        wk_slot[k] = b + t;
-        wk_f[k] = f;
+    uint64_t mk[SP_KMAX][SPL];
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < SP_KMAX; ++k)
+#pragma unroll
+      for (int j = 0; j < SPL; ++j) {
+        const int l = t - k * delta, i = tid + j * SP_THREADS;
+        if (fk[k][j]) ring_b[(size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP + i] = 0;
+        mk[k][j] = __ballot(fk[k][j] != 0u);
+        total += __popcll(mk[k][j]);
       }
-      // prefetch the following level and the next pre-check-active level;
-      // issued after this level's own loads were consumed, so no wait of
-      // this level covers them
-      pn_level = -1;
-      if (ne > nb) {
-        pn_level = L + 1;
-        if (nb + t < ne) {
-          pn = load_head(head, nb + t);
-          pnf = sflag[nb + t];
+    if (total) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&s_nwork[cur], total);
+      base = __shfl(base, 0);
+      const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+      for (int k = 0; k < SP_KMAX; ++k)
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) {
+          const int i = tid + j * SP_THREADS;
+          if (fk[k][j])
+            wk[base + __popcll(mk[k][j] & below)] = (uint32_t)i | ((uint32_t)k << 10) | (fk[k][j] << 14);
+          base += __popcll(mk[k][j]);
+        }
+    }
+    __syncthreads();
+    // --- evaluate ---
+    const int nwork = s_nwork[cur];
+    if (tid == 0) s_nwork[cur ^ 1] = 0;  // next iteration's list
+    bool marked = false;
+    // one work entry: its slot record r, loaded by the caller
+    auto evaluate = [&](const StrainSlot& r, uint32_t e) __attribute__((always_inline)) {
+      const int idx = (int)(e & 1023u), k = (int)((e >> 10) & 15u);
+      const uint32_t f = e >> 14;
+      const int l = t - k * delta;
+      const int s = lvp(l) + idx;
+      const bool tri = r.v[2] >= 0;
+      dvec4 a[3] = {r.vel[0], r.vel[1], tri ? r.vel[2] : dvec4{}};
+      uint32_t wmask = 0u;
+      int edges = 0, degen = 0;
+      bool g;
+      auto average = [&](int i, int j) {
+        double vt[3] = {a[i].x + a[j].x, a[i].y + a[j].y, a[i].z + a[j].z};
+        vt[0] = 0.5 * vt[0];
+        vt[1] = 0.5 * vt[1];
+        vt[2] = 0.5 * vt[2];
+        a[i] = dvec4{vt[0], vt[1], vt[2], 0.0};
+        a[j] = a[i];
+        wmask |= (1u << i) | (1u << j);
+        ++edges;
+        if (g) ++degen;
+      };
+      if (edge_over_limit_l(r.x, r.x + 3, a[0], a[1], r.lold[0], r.L0[0], dt, &g)) average(0, 1);
+      if (tri) {
+        if (edge_over_limit_l(r.x + 3, r.x + 6, a[1], a[2], r.lold[1], r.L0[1], dt, &g))
+          average(1, 2);
+        if (edge_over_limit_l(r.x + 6, r.x, a[2], a[0], r.lold[2], r.L0[2], dt, &g))
+          average(2, 0);
+      }
+      // every register of the record has arrived before the first store: a
+      // load left outstanding (the third vertex of a bond) would make the
+      // next iteration wait for this iteration's stores as well
+      asm volatile("" ::"v"(r.x[6]), "v"(r.lold[2]), "v"(r.L0[2]), "v"(r.ns[2]), "v"(r.nl[2]),
+                   "v"(r.vel[2].x), "v"(r.vel[2].w), "v"(r.vel[1].w), "v"(r.vel[0].w));
+      const int np = tri ? 3 : 2;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        if (p >= np) break;
+        const bool wrote = (wmask >> p) & 1u;
+        if (wrote) {
+          slots[s].vel[p] = vel_with_origin(a[p], s);
+        } else if (!((f >> p) & 1u)) {
+          continue;
+        }
+        const int ns = r.ns[p];
+        const int ts = ns >> 2, tp = ns & 3, tl = r.nl[p];
+        if (ns < 0 || ts == s) continue;
+        const int origin = wrote ? s : origin_of(r.vel[p]);
+        if (!wrote && ts == origin) continue;  // back at the writer: its copy is current
+        const bool wrap = tl < l;               // first visit, next sweep
+        if (wrap && k == K - 1) continue;
+        slots[ts].vel[tp] = vel_with_origin(a[p], origin);
+        if (!wrap && tl - l >= SP_RING) {       // far ahead (one sweep in flight)
+          spre_b[4 * (size_t)ts + tp] = 1;
+          if (use_preb) bit_set(preb, tl);
+        } else {
+          const int ti = ts - lvp(tl);
+          atomicOr(&ring[tl & (SP_RING - 1)][ti >> 2], (1u << tp) << (8 * (ti & 3)));
+          marked = true;
         }
       }
-      const int P = bit_next(preb, L + 2, n_levels);
-      if (P < n_levels && P != pq_level) {
-        const int pb = lvp(P), pe = lvp(P + 1);
-        pq_level = -1;
-        if (pe - pb <= STRAIN_TAB) {
-          pq_level = P;
-          if (pb + t < pe) {
-            pq = load_head(head, pb + t);
-            pqf = sflag[pb + t];
-          }
+      if (edges) {
+        e_any |= 1u << k;
+        if (k == K - 1) e_last += edges;
+        d_acc += degen;
+        if (k < K - 1) {  // evaluate again in the next sweep
+          atomicOr(&ring[l & (SP_RING - 1)][idx >> 2], 8u << (8 * (idx & 3)));
+          marked = true;
         }
       }
-      __syncthreads();
-      const int nwork = s_nwork[cur];
-      if (t == 0) s_nwork[cur ^ 1] = 0;  // next level's list; last read before the previous barrier
-      for (int i = t; i < nwork; i += STRAIN_THREADS) {
-        strain_eval(wk_head[i], wk_f[i], body, wk_slot[i], L + 1, av, M, dt, C);
-        ++evals;
-      }
-    } else {
-      // large level (or no LDS bits): flag words from memory, heads loaded here
-      for (int s = b + t; s < e; s += STRAIN_THREADS) {
-        const uint32_t f = sflag[s];
-        if (f == 0u) continue;
-        const StrainHead h = head[s];
-        strain_eval(h, f, body, s, L + 1, av, M, dt, C);
-        ++evals;
-      }
-      pn_level = -1;
+    };
+    auto slot_of_entry = [&](uint32_t e) {
+      return lvp(t - (int)((e >> 10) & 15u) * delta) + (int)(e & 1023u);
+    };
+    // first round: this lane's entry and the first sweep's next pre-check
+    // word are loaded together, the word folded into the ring before any
+    // store of this iteration
+    const PreWords pw = pre_word(t + 1);
+    const bool have = tid < nwork;
+    const uint32_t e0 = have ? wk[tid] : 0u;
+    StrainSlot r0;
+    if (have) r0 = slots[slot_of_entry(e0)];
+    fold_pre(t + 1, pw);  // (no word beyond the last level)
+    if (have) evaluate(r0, e0);
+    for (int q = tid + SP_THREADS; q < nwork; q += SP_THREADS) {
+      const uint32_t e = wk[q];
+      const StrainSlot r = slots[slot_of_entry(e)];
+      evaluate(r, e);
     }
-    if (t == 0) ++s_levels;
+    // every mark is consumed within delta iterations
+    if (__ballot(marked) && lane == 0) s_last_mark = t;
+    if (tid == 0) evals += nwork;
+    ++iters;
     __syncthreads();
     cur ^= 1;
-    L = use_lds ? bit_next(actb, L + 1, n_levels) : L + 1;
+    int tn = t + 1;
+    if (use_preb && t >= s_last_mark + delta) {
+      // nothing pending: only the first sweep's pre-check levels remain
+      tn = bit_next(preb, t + 1, n_levels);
+      if (tn >= n_levels) break;
+    }
+    if (tn != t + 1) {  // skipped: the pre-check word of the new level
+      fold_pre(tn, pre_word(tn));
+      __syncthreads();
+    }
+    t = tn;
   }
-  if (evals) atomicAdd(&s_evals, evals);
+  // totals
+  {
+    uint32_t any = e_any;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) any |= __shfl_xor(any, o);
+    const int el = wave_sum(e_last), dg = wave_sum(d_acc);
+    if (lane == 0) {
+      s_any[wave] = any;
+      s_last_edges[wave] = el;
+      s_degen[wave] = dg;
+    }
+  }
   __syncthreads();
-  if (t == 0) {
-    counters[CNT_STRAIN_EDGES] = s_edges;
-    counters[CNT_STRAIN_DEGEN] += s_degen;
-    counters[CNT_STRAIN_SWEEPS] += 1;
-    counters[CNT_STRAIN_LEVELS] += s_levels;
-    counters[CNT_STRAIN_EVALS] += s_evals;
-    counters[CNT_STRAIN_FLAG] = s_flag ? 0 : -1;  // next check decides; no edge -> finished
+  if (tid == 0) {
+    uint32_t any = 0u;
+    int last = 0, degen = 0;
+    for (int w = 0; w < SP_THREADS / 64; ++w) {
+      any |= s_any[w];
+      last += s_last_edges[w];
+      degen += s_degen[w];
+    }
+    int nz = 0;
+    while (nz < K && ((any >> nz) & 1u)) ++nz;
+    const int niter = nz < K ? nz + 1 : K;  // sweeps the sequential loop runs
+    counters[CNT_STRAIN_EDGES] = nz < K ? 0 : last;
+    counters[CNT_STRAIN_DEGEN] += degen;
+    counters[CNT_STRAIN_SWEEPS] += niter;
+    counters[CNT_STRAIN_LEVELS] += iters;
+    counters[CNT_STRAIN_EVALS] += (int)evals;
+    counters[CNT_STRAIN_FLAG] = (nz == K) ? 0 : -1;  // all K wrote: the loop goes on
   }
 }
 
-hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps, int* sweeps_out) {
-  (void)sweeps_out;
+hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps) {
   const int n = S.n_sched;
   if (S.ne == 0 || n == 0 || max_sweeps <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_strain_pack, dim3((n + 255) / 256), dim3(256), 0, st, S.shead, S.sbody,
-                     S.x_old, n);
-  for (int s = 0; s < max_sweeps; ++s) {
+  // sweeps in flight: marks pending at one time cover K * delta - 1 levels
+  const int kfit = std::max(1, SP_RING / std::max(1, S.strain_delta));
+  for (int k0 = 0; k0 < max_sweeps;) {
+    const int K = std::min({SP_KMAX, kfit, max_sweeps - k0});
+    hipLaunchKernelGGL(k_strain_pack, dim3((n + 255) / 256), dim3(256), 0, st, S.sslot, S.x_old,
+                       S.av, n, k0 == 0 ? 1 : 0);
     CS_HIP_RET(hipMemsetAsync(S.lvl_active, 0, (size_t)S.n_levels, st));
-    hipLaunchKernelGGL(k_strain_check, dim3((n + 255) / 256), dim3(256), 0, st, S.shead, S.sbody,
-                       S.av, dt, n, S.sflag, S.lvl_active, S.counters);
-    if (S.n_levels <= STRAIN_LP_MAX)
-      hipLaunchKernelGGL(k_strain_sweep<true>, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
-                         S.n_levels, S.shead, S.snl, S.sbody, S.sflag, S.lvl_active, S.av, dt,
+    hipLaunchKernelGGL(k_strain_check, dim3((n + 255) / 256), dim3(256), 0, st, S.sslot, dt, n,
+                       S.spre, S.lvl_active, S.counters);
+    if (S.n_levels <= SP_LP_MAX)
+      hipLaunchKernelGGL(k_strain_pipe<true>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
+                         S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
                          S.counters);
     else
-      hipLaunchKernelGGL(k_strain_sweep<false>, dim3(1), dim3(STRAIN_THREADS), 0, st, S.lvl_ptr,
-                         S.n_levels, S.shead, S.snl, S.sbody, S.sflag, S.lvl_active, S.av, dt,
+      hipLaunchKernelGGL(k_strain_pipe<false>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
+                         S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
                          S.counters);
+    hipLaunchKernelGGL(k_strain_writeback, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.sslot,
+                       S.slast, S.av, S.nv);
+    k0 += K;
   }
   return hipGetLastError();
 }
