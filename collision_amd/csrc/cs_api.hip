@@ -98,6 +98,7 @@ void free_dev(DevState& S) {
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
   dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist);
+  dfree(S.dpre); dfree(S.rflag); dfree(S.rlist);
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
   dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
@@ -320,6 +321,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // phase appends to them, and overflows are detected afterwards (the
     // narrow phase only appends records and performs idempotent zone
     // merges, so it can be run again).
+    launch_clear_ranges(h->st, S);
     launch_mark_dirty(h->st, S);
     HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
     mark(h, &T.t0);
@@ -664,6 +666,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.dirty_v, nv));
   HIPCHK(h, dalloc(&S.dirty_e, ne));
   HIPCHK(h, dalloc(&S.dlist, ne));
+  HIPCHK(h, dalloc(&S.dpre, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.rflag, (2 * ne - 1 + 63) / 64)); HIPCHK(h, dalloc(&S.rlist, (2 * ne - 1 + 63) / 64));
   HIPCHK(h, dalloc(&S.zmark, nv)); HIPCHK(h, dalloc(&S.zlist, nv));
   HIPCHK(h, dalloc(&S.zkey64, nv)); HIPCHK(h, dalloc(&S.zkey64_tmp, nv));
   HIPCHK(h, dalloc(&S.zkey, nv));
@@ -690,6 +694,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
   HIPCHK(h, hipMemset(S.has, 0, nv));
   HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
+  HIPCHK(h, hipMemset(S.dpre, 0, 2 * (size_t)ne - 1));
+  HIPCHK(h, hipMemset(S.rflag, 0, sizeof(int32_t) * ((2 * (size_t)ne - 1 + 63) / 64)));
   h->assembled = true;
   h->state_set = false;
   return CS_OK;
@@ -764,6 +770,9 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   h->prev = PrevPass();
   hipEvent_t t0, t_strain0, t_end;
   mark(h, &t0);
+  // the dirty-range marks of the last pass are cleared while their count is
+  // still in the counter block
+  if (S.ne >= 2) launch_clear_ranges(h->st, S);
   HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
   launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
   HIPCHK(h, hipMemsetAsync(S.zmark, 0, sizeof(int32_t) * (size_t)S.nv, h->st));

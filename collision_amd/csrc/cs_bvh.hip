@@ -469,6 +469,10 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
                                                     const uint8_t* __restrict__ dirty_v, int ne,
                                                     uint8_t* __restrict__ dirty_e,
                                                     int32_t* __restrict__ dlist,
+                                                    const int32_t* __restrict__ enode,
+                                                    uint8_t* __restrict__ dpre,
+                                                    int32_t* __restrict__ rflag,
+                                                    int32_t* __restrict__ rlist,
                                                     int* __restrict__ counters) {
   __shared__ int s_cnt, s_base;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -477,6 +481,11 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
     const int4 t = ev[e];
     d = dirty_v[t.x] | dirty_v[t.y] | ((t.z >= 0) ? dirty_v[t.z] : 0);
     dirty_e[e] = d ? 1 : 0;
+    if (d) {  // the leaf, and its 64-node range of the pre-order layout
+      const int p = enode[e];
+      dpre[p] = 1;
+      if (atomicExch(&rflag[p >> 6], 1) == 0) rlist[atomicAdd(&counters[CNT_DRANGES], 1)] = p >> 6;
+    }
   }
   const int slot = block_append(d, &counters[CNT_DIRTY], &s_cnt, &s_base);
   if (d) dlist[slot] = e;
@@ -571,85 +580,235 @@ __global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebo
 // elements is taken from the one earlier in Morton order.  emit = 0: count
 // only (previous boxes, into CNT_OLD64); emit = 1: count into CNT_NEW64 and
 // append non-rejected pairs to qpairs.
-__global__ void __launch_bounds__(TRAV_THREADS) k_query_dirty(
-    const bvh_node* __restrict__ nodes, const dbox* __restrict__ ebox,
-    const int4* __restrict__ ev, const uint8_t* __restrict__ eflags,
-    const uint8_t* __restrict__ dirty_e, const int32_t* __restrict__ dlist,
-    const int32_t* __restrict__ enode, int n, int emit, int2* __restrict__ qpairs, int qcap,
-    int* __restrict__ counters) {
-  __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
-  __shared__ int s_base;
-  __shared__ unsigned long long s_w[TRAV_THREADS / 64];
-  __shared__ int s_off[TRAV_THREADS];
-  const int t = threadIdx.x;
-  const int ndirty = counters[CNT_DIRTY];
-  const int rounds = (ndirty + (int)(gridDim.x * TRAV_THREADS) - 1) / (int)(gridDim.x * TRAV_THREADS);
-  for (int round = 0; round < rounds; ++round) {
-  const int k = (round * (int)gridDim.x + xcd_block(blockIdx.x, gridDim.x)) * TRAV_THREADS + t;
+//
+// One wavefront per 64-node range of the pre-order layout that holds dirty
+// leaves: the range's dirty leaves are neighbours in Morton order, so the
+// union of their boxes is small.  The wavefront descends the tree breadth
+// first with that union box, one level per step with every lane loading one
+// node of the frontier, collects the leaves the union box overlaps, and each
+// lane then tests its own element against them with the exact boxes.  The
+// number of dependent loads is the tree depth rather than the length of a
+// single-lane walk.  A frontier or leaf list too large for LDS falls back to
+// one stackless walk per lane.
+constexpr int QW_FCAP = 1024;  // frontier nodes of one level
+constexpr int QW_LCAP = 1024;  // leaves collected by one range
+
+__device__ inline void flush_pairs(int2* buf, int nbuf, int2* __restrict__ qpairs, int qcap,
+                                   int* __restrict__ counters) {
+  // exclusive scan of nbuf over the wavefront, one atomic per wavefront
+  const int lane = threadIdx.x & 63;
+  int incl = nbuf;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  const int total = __shfl(incl, 63);
+  if (total == 0) return;
+  int base = 0;
+  if (lane == 63) base = atomicAdd(&counters[CNT_QPAIRS], total);
+  base = __shfl(base, 63) + incl - nbuf;
+  for (int q = 0; q < nbuf; ++q)
+    if (base + q < qcap) qpairs[base + q] = buf[q * 64 + lane];
+}
+
+__global__ void __launch_bounds__(64) k_query_ranges(
+    const bvh_node* __restrict__ nodes, const int32_t* __restrict__ nright,
+    const dbox* __restrict__ ebox, const int4* __restrict__ ev,
+    const uint8_t* __restrict__ eflags, const uint8_t* __restrict__ dirty_e,
+    const uint8_t* __restrict__ dpre, const int32_t* __restrict__ rlist, int npre, int emit,
+    int2* __restrict__ qpairs, int qcap, int* __restrict__ counters) {
+  __shared__ int32_t fr[2][QW_FCAP];
+  __shared__ int2 lv[QW_LCAP];  // (pre-order index, element)
+  // the range's dirty elements (the queries), compacted
+  __shared__ dbox qb[64];
+  __shared__ int4 qt[64];
+  __shared__ int32_t qs[64], qe[64];
+  __shared__ uint8_t qf[64];
+  __shared__ int2 buf[TRAV_BUF * 64];
+  const int lane = threadIdx.x;
+  const int nr = counters[CNT_DRANGES];
   unsigned long long cand = 0;
-  int nbuf = 0;
-  if (k < ndirty && n > 1) {
-    const int e = dlist[k];
-    const int si = enode[e];  // pre-order index of e's leaf (increasing with Morton order)
-    const bvh_node me = nodes[si];
-    const dbox mb = ebox[e];
-    const int4 mt = ev[e];
-    const uint8_t mf = eflags[e];
-    int nd = 0;  // root
-    while (nd >= 0) {
-      const bvh_node node = nodes[nd];
-      if (!f32_overlap(node, me.lo, me.hi)) {
-        nd = node.rope;
-        continue;
+  // ranges in block order: the first nr blocks are busy, and the hardware's
+  // round-robin placement spreads them over all XCDs
+  for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
+    const int si = 64 * rlist[ri] + lane;  // this lane's node
+    const bool act = si < npre && dpre[si];
+    bvh_node me{};
+    dbox mb{};
+    int4 mt{};
+    uint8_t mf = 0;
+    int e = -1;
+    float ulo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, uhi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    if (act) {
+      me = nodes[si];
+      e = me.left;
+      mb = ebox[e];
+      mt = ev[e];
+      mf = eflags[e];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ulo[d] = me.lo[d];
+        uhi[d] = me.hi[d];
       }
-      if (node.left < 0) {  // internal: its first child is the next record
-        ++nd;
-        continue;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ulo[d] = fminf(ulo[d], __shfl_xor(ulo[d], o));
+        uhi[d] = fmaxf(uhi[d], __shfl_xor(uhi[d], o));
       }
-      const int j = nd;
-      nd = node.rope;
-      if (j == si) continue;
-      const int e2 = node.left;
-      if (dirty_e[e2] && j < si) continue;  // counted from e2
-      const dbox ob = ebox[e2];
-      if (!f64_overlap(ob, mb)) continue;
+    // breadth-first descent with the union box
+    if (lane == 0) fr[0][0] = 0;
+    __syncthreads();
+    int cur = 0, nfr = 1, nlv = 0;
+    bool overflow = false;
+    const uint64_t below = (1ull << lane) - 1ull;
+    while (nfr > 0) {
+      int nnext = 0;
+      for (int b = 0; b < nfr; b += 64) {
+        const int i = b + lane;
+        bool kl = false, ki = false;
+        int node = 0, c2 = 0, left = -1;
+        if (i < nfr) {
+          node = fr[cur][i];
+          const bvh_node N = nodes[node];
+          const int r2 = nright[node];
+          if (f32_overlap(N, ulo, uhi)) {
+            if (N.left >= 0) {
+              kl = true;
+              left = N.left;
+            } else {
+              ki = true;
+              c2 = r2;
+            }
+          }
+        }
+        const uint64_t ml = __ballot(kl), mi = __ballot(ki);
+        if (kl) {
+          const int q = nlv + __popcll(ml & below);
+          if (q < QW_LCAP) lv[q] = make_int2(node, left);
+        }
+        if (ki) {
+          const int q = nnext + 2 * __popcll(mi & below);
+          if (q + 1 < QW_FCAP) {
+            fr[cur ^ 1][q] = node + 1;
+            fr[cur ^ 1][q + 1] = c2;
+          }
+        }
+        nlv += __popcll(ml);
+        nnext += 2 * __popcll(mi);
+      }
+      __syncthreads();
+      if (nlv > QW_LCAP || nnext > QW_FCAP) {
+        overflow = true;
+        break;
+      }
+      cur ^= 1;
+      nfr = nnext;
+    }
+    int nbuf = 0;
+    // a box-overlapping pair (query element qel at pre-order qsi, leaf j)
+    auto take = [&](int qsi, int qel, const int4& qtop, uint8_t qfl, int j, int e2,
+                    const int4& ot, uint8_t of) {
       ++cand;
-      if (!emit) continue;
-      const int4 ot = ev[e2];
-      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
-      if (rej) continue;
-      const int2 pr = make_int2(min(e, e2), max(e, e2));
+      if (!emit) return;
+      const bool rej = (qel > e2) ? pair_rejected(qtop, ot, qfl) : pair_rejected(ot, qtop, of);
+      if (rej) return;
+      const int2 pr = make_int2(min(qel, e2), max(qel, e2));
       if (nbuf < TRAV_BUF) {
-        buf[nbuf * TRAV_THREADS + t] = pr;
+        buf[nbuf * 64 + lane] = pr;
         ++nbuf;
       } else {
         const int slot = atomicAdd(&counters[CNT_QPAIRS], 1);
         if (slot < qcap) qpairs[slot] = pr;
       }
+      (void)qsi;
+    };
+    if (!overflow) {
+      // lanes hold the collected leaves, 64 at a time; the queries are
+      // broadcast from LDS
+      const uint64_t am = __ballot(act);
+      const int na = __popcll(am);
+      if (act) {
+        const int q = __popcll(am & below);
+        qb[q] = mb;
+        qt[q] = mt;
+        qs[q] = si;
+        qe[q] = e;
+        qf[q] = mf;
+      }
+      __syncthreads();
+      for (int b = 0; b < nlv; b += 64) {
+        const bool have = b + lane < nlv;
+        int j = -1, e2 = 0;
+        dbox ob{};
+        int4 ot{};
+        uint8_t of = 0, od = 0;
+        if (have) {
+          const int2 L = lv[b + lane];
+          j = L.x;
+          e2 = L.y;
+          ob = ebox[e2];
+          ot = ev[e2];
+          of = eflags[e2];
+          od = dirty_e[e2];
+        }
+        for (int q = 0; q < na; ++q) {
+          const int qsi = qs[q];
+          const dbox B = qb[q];
+          // a pair of two dirty elements is taken from the one earlier in
+          // pre-order; the overlap test without branches (AABB::isCollid)
+          const bool hit = have && j != qsi && !(od && j < qsi) &&
+                           ((ob.lo[0] <= B.hi[0]) & (ob.hi[0] >= B.lo[0]) & (ob.lo[1] <= B.hi[1]) &
+                            (ob.hi[1] >= B.lo[1]) & (ob.lo[2] <= B.hi[2]) & (ob.hi[2] >= B.lo[2]));
+          if (hit) take(qsi, qe[q], qt[q], qf[q], j, e2, ot, of);
+        }
+      }
+    } else if (act) {
+      // stackless walk from the root with this lane's leaf box
+      int nd = 0;
+      while (nd >= 0) {
+        const bvh_node node = nodes[nd];
+        if (!f32_overlap(node, me.lo, me.hi)) {
+          nd = node.rope;
+          continue;
+        }
+        if (node.left < 0) {  // internal: its first child is the next record
+          ++nd;
+          continue;
+        }
+        const int j = nd;
+        nd = node.rope;
+        const int e2 = node.left;
+        if (j == si) continue;
+        const uint8_t od = dirty_e[e2];
+        if (od && j < si) continue;  // counted from e2
+        if (!f64_overlap(ebox[e2], mb)) continue;
+        take(si, e, mt, mf, j, e2, ev[e2], eflags[e2]);
+      }
     }
-  }
-  const unsigned long long c = block_sum_u64(cand, s_w);
-  if (t == 0 && c)
-    atomicAdd((unsigned long long*)(counters + (emit ? CNT_NEW64 : CNT_OLD64)), c);
-  if (!emit) continue;
-  // append the buffered pairs: exclusive scan of nbuf over the block
-  s_off[t] = nbuf;
-  __syncthreads();
-  for (int off = 1; off < TRAV_THREADS; off <<= 1) {
-    const int add = (t >= off) ? s_off[t - off] : 0;
-    __syncthreads();
-    s_off[t] += add;
+    if (emit) flush_pairs(buf, nbuf, qpairs, qcap, counters);
     __syncthreads();
   }
-  if (t == TRAV_THREADS - 1) {
-    const int total = s_off[t];
-    s_base = total ? atomicAdd(&counters[CNT_QPAIRS], total) : 0;
-  }
-  __syncthreads();
-  const int base = s_base + s_off[t] - nbuf;
-  for (int q = 0; q < nbuf; ++q)
-    if (base + q < qcap) qpairs[base + q] = buf[q * TRAV_THREADS + t];
-  __syncthreads();
+  unsigned long long c = cand;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0 && c) atomicAdd((unsigned long long*)(counters + (emit ? CNT_NEW64 : CNT_OLD64)), c);
+}
+
+// clear the dirty marks of the ranges listed by the previous pass
+__global__ void __launch_bounds__(64) k_clear_ranges(const int32_t* __restrict__ rlist, int npre,
+                                                     uint8_t* __restrict__ dpre,
+                                                     int32_t* __restrict__ rflag,
+                                                     const int* __restrict__ counters) {
+  const int nr = counters[CNT_DRANGES];
+  for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
+    const int r = rlist[ri];
+    const int p = 64 * r + (int)threadIdx.x;
+    if (p < npre) dpre[p] = 0;
+    if (threadIdx.x == 0) rflag[r] = 0;
   }
 }
 
@@ -732,7 +891,14 @@ __global__ void __launch_bounds__(256) k_carry_hp(const int4* __restrict__ prev,
 
 void launch_mark_dirty(hipStream_t st, DevState& S) {
   hipLaunchKernelGGL(k_mark_dirty, dim3((S.ne + 255) / 256), dim3(256), 0, st, S.ev, S.dirty_v,
-                     S.ne, S.dirty_e, S.dlist, S.counters);
+                     S.ne, S.dirty_e, S.dlist, S.leaf_of, S.dpre, S.rflag, S.rlist, S.counters);
+}
+
+void launch_clear_ranges(hipStream_t st, DevState& S) {
+  const int npre = 2 * S.ne - 1;
+  hipLaunchKernelGGL(k_clear_ranges, dim3(std::min((npre + 63) / 64, 2048)), dim3(64), 0, st,
+                     S.rlist, npre, S.dpre, S.rflag, S.counters);
+  (void)hipMemsetAsync(S.counters + CNT_DRANGES, 0, sizeof(int), st);
 }
 
 void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
@@ -748,9 +914,11 @@ void launch_grow_dirty(hipStream_t st, DevState& S) {
 }
 
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
-  hipLaunchKernelGGL(k_query_dirty, dim3(std::min((S.ne + TRAV_THREADS - 1) / TRAV_THREADS, 256)),
-                     dim3(TRAV_THREADS), 0, st, S.nodes, S.ebox, S.ev, S.eflags,
-                     S.dirty_e, S.dlist, S.leaf_of, S.ne, emit, S.qpairs, S.qpair_cap, S.counters);
+  if (S.ne < 2) return;
+  const int npre = 2 * S.ne - 1;
+  hipLaunchKernelGGL(k_query_ranges, dim3(std::min((npre + 63) / 64, 4096)), dim3(64), 0, st,
+                     S.nodes, S.nright, S.ebox, S.ev, S.eflags, S.dirty_e, S.dpre, S.rlist, npre,
+                     emit, S.qpairs, S.qpair_cap, S.counters);
 }
 
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev) {

@@ -46,6 +46,7 @@ enum {
   CNT_OLD64 = 18,        // candidates touching dirty elements, previous boxes
   CNT_NEW64 = 20,        // candidates touching dirty elements, current boxes
   CNT_ZLIST = 22,        // vertices in the zone member list (kept for the whole step)
+  CNT_DRANGES = 23,      // tree ranges holding dirty leaves (kept until the next pass clears them)
   CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
   CNT_INTS = 32
 };
@@ -113,7 +114,10 @@ struct DevState {
   int n_top_start = 0;           // (built once per LBVH build)
   uint32_t* scan_part = nullptr; // partial sums of the device scan
   size_t scan_part_words = 0;
-  int32_t* leaf_of = nullptr;    // element -> sorted leaf index
+  int32_t* leaf_of = nullptr;    // element -> pre-order index of its leaf
+  // dirty leaves by pre-order index, and the 64-node ranges holding them
+  uint8_t* dpre = nullptr;
+  int32_t *rflag = nullptr, *rlist = nullptr;
   // candidate stream (current and previous pass)
   int2* pairs = nullptr;
   int2* pairs_prev = nullptr;
@@ -179,6 +183,7 @@ hipError_t refit_bvh(hipStream_t st, DevState& S);
 void launch_traverse(hipStream_t st, DevState& S, int cap);
 // incremental passes
 void launch_mark_dirty(hipStream_t st, DevState& S);
+void launch_clear_ranges(hipStream_t st, DevState& S);
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
 void launch_grow_dirty(hipStream_t st, DevState& S);
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);

@@ -531,7 +531,11 @@ __global__ void __launch_bounds__(NARROW_THREADS, 3)
              ZoneList Z, hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp,
              int hp_cap, int* __restrict__ counters) {
   const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
-  const int64_t gl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * NARROW_THREADS + threadIdx.x;
+  // XCD-contiguous shares of the pairs when the grid matches the work; with a
+  // grid sized from a guess the busy blocks are the first ones, and the
+  // hardware's round-robin order spreads them over all XCDs
+  const int b = npairs_dev ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+  const int64_t gl = (int64_t)b * NARROW_THREADS + threadIdx.x;
   if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
   narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
 }
