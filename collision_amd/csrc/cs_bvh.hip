@@ -620,10 +620,15 @@ __global__ void __launch_bounds__(64) k_query_ranges(
     int2* __restrict__ qpairs, int qcap, int* __restrict__ counters) {
   __shared__ int32_t fr[2][QW_FCAP];
   __shared__ int2 lv[QW_LCAP];  // (pre-order index, element)
-  // the range's dirty elements (the queries), compacted
-  __shared__ dbox qb[64];
+  // the range's dirty elements (the queries), compacted: box and pre-order
+  // index in one 64 B record (one LDS round trip per query), the rest apart
+  struct QueryRec {
+    dbox b;
+    int32_t si, pad[3];
+  };
+  __shared__ QueryRec qr[64];
   __shared__ int4 qt[64];
-  __shared__ int32_t qs[64], qe[64];
+  __shared__ int32_t qe[64];
   __shared__ uint8_t qf[64];
   __shared__ int2 buf[TRAV_BUF * 64];
   const int lane = threadIdx.x;
@@ -733,9 +738,9 @@ __global__ void __launch_bounds__(64) k_query_ranges(
       const int na = __popcll(am);
       if (act) {
         const int q = __popcll(am & below);
-        qb[q] = mb;
+        qr[q].b = mb;
+        qr[q].si = si;
         qt[q] = mt;
-        qs[q] = si;
         qe[q] = e;
         qf[q] = mf;
       }
@@ -755,15 +760,23 @@ __global__ void __launch_bounds__(64) k_query_ranges(
           of = eflags[e2];
           od = dirty_e[e2];
         }
+        // overlaps first (a bit per query), rejects and output afterwards
+        uint64_t hits = 0;
+#pragma unroll 2
         for (int q = 0; q < na; ++q) {
-          const int qsi = qs[q];
-          const dbox B = qb[q];
+          const QueryRec Q = qr[q];
           // a pair of two dirty elements is taken from the one earlier in
           // pre-order; the overlap test without branches (AABB::isCollid)
-          const bool hit = have && j != qsi && !(od && j < qsi) &&
-                           ((ob.lo[0] <= B.hi[0]) & (ob.hi[0] >= B.lo[0]) & (ob.lo[1] <= B.hi[1]) &
-                            (ob.hi[1] >= B.lo[1]) & (ob.lo[2] <= B.hi[2]) & (ob.hi[2] >= B.lo[2]));
-          if (hit) take(qsi, qe[q], qt[q], qf[q], j, e2, ot, of);
+          const bool hit = (j != Q.si) & !(od && j < Q.si) &
+                           (ob.lo[0] <= Q.b.hi[0]) & (ob.hi[0] >= Q.b.lo[0]) &
+                           (ob.lo[1] <= Q.b.hi[1]) & (ob.hi[1] >= Q.b.lo[1]) &
+                           (ob.lo[2] <= Q.b.hi[2]) & (ob.hi[2] >= Q.b.lo[2]);
+          hits |= (uint64_t)(have && hit) << q;
+        }
+        while (hits) {
+          const int q = __builtin_ctzll(hits);
+          hits &= hits - 1;
+          take(qr[q].si, qe[q], qt[q], qf[q], j, e2, ot, of);
         }
       }
     } else if (act) {
