@@ -705,36 +705,45 @@ __global__ void __launch_bounds__(SP_THREADS)
   if (t < n_levels) fold_pre(t, pre_word(t));
   __syncthreads();
   while (t < tend) {
-    // --- scan: lane tid looks at slot tid of every level in flight ---
-    // (all offsets first, then all marks: two LDS round trips)
+    // --- scan: lane tid looks at slots 2 tid, 2 tid + 1 of every level in
+    // flight.  Level offsets: lane 2k (2k + 1) reads the first (last) offset
+    // of sweep k's level, broadcast with readlane; then one 16-bit LDS read
+    // per level.
+    static_assert(SPL == 2, "two adjacent slots per lane");
     int lb[SP_KMAX], ln[SP_KMAX];
+    {
+      const int kk = lane >> 1, l = t - kk * delta;
+      const bool ok = kk < K && l >= 0 && l < n_levels;
+      const int v = ok ? lvp(l + (lane & 1)) : 0;
 #pragma unroll
-    for (int k = 0; k < SP_KMAX; ++k) {
-      const int l = t - k * delta;
-      const bool ok = k < K && l >= 0 && l < n_levels;
-      const int b = ok ? lvp(l) : 0, e = ok ? lvp(l + 1) : 0;
-      lb[k] = b;
-      ln[k] = e - b;
+      for (int k = 0; k < SP_KMAX; ++k) {
+        lb[k] = __builtin_amdgcn_readlane(v, 2 * k);
+        ln[k] = __builtin_amdgcn_readlane(v, 2 * k + 1) - lb[k];
+      }
     }
     uint32_t fk[SP_KMAX][SPL];
 #pragma unroll
-    for (int k = 0; k < SP_KMAX; ++k)
-#pragma unroll
-      for (int j = 0; j < SPL; ++j) {
-        const int l = t - k * delta, i = tid + j * SP_THREADS;
-        fk[k][j] = i < ln[k] ? ring_b[(size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP + i] : 0u;
-      }
+    for (int k = 0; k < SP_KMAX; ++k) {
+      const int l = t - k * delta;
+      const uint16_t* row =
+          reinterpret_cast<const uint16_t*>(ring_b + (size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP);
+      const uint32_t w = 2 * tid < ln[k] ? row[tid] : 0u;
+      fk[k][0] = w & 0xffu;
+      fk[k][1] = 2 * tid + 1 < ln[k] ? (w >> 8) : 0u;
+    }
     uint64_t mk[SP_KMAX][SPL];
     int total = 0;
 #pragma unroll
-    for (int k = 0; k < SP_KMAX; ++k)
+    for (int k = 0; k < SP_KMAX; ++k) {
+      const int l = t - k * delta;
+      if (fk[k][0] | fk[k][1])
+        reinterpret_cast<uint16_t*>(ring_b + (size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP)[tid] = 0;
 #pragma unroll
       for (int j = 0; j < SPL; ++j) {
-        const int l = t - k * delta, i = tid + j * SP_THREADS;
-        if (fk[k][j]) ring_b[(size_t)(l & (SP_RING - 1)) * STRAIN_LEVEL_CAP + i] = 0;
         mk[k][j] = __ballot(fk[k][j] != 0u);
         total += __popcll(mk[k][j]);
       }
+    }
     if (total) {
       int base = 0;
       if (lane == 0) base = atomicAdd(&s_nwork[cur], total);
@@ -744,7 +753,7 @@ __global__ void __launch_bounds__(SP_THREADS)
       for (int k = 0; k < SP_KMAX; ++k)
 #pragma unroll
         for (int j = 0; j < SPL; ++j) {
-          const int i = tid + j * SP_THREADS;
+          const int i = 2 * tid + j;
           if (fk[k][j])
             wk[base + __popcll(mk[k][j] & below)] = (uint32_t)i | ((uint32_t)k << 10) | (fk[k][j] << 14);
           base += __popcll(mk[k][j]);
