@@ -1,0 +1,94 @@
+"""CPU model of the pipelined strain-limit sweep against the sequential sweep.
+
+The device kernel k_strain_pipe runs up to ten reduceSuperelastOnce sweeps
+(dcollid.cpp:486-560) at once, sweep k+1 trailing sweep k by `delta` levels of
+a span-bounded schedule, and evaluates only the slots whose result can
+change.  tests/cpp/strain_pipeline_model.c restates that scheme on the CPU and
+compares it bit for bit with the sequential loop (dcollid.cpp:583-596) on the
+meshes below; two planted mistakes (sweeps run closer than the schedule
+allows; tokens of non-writing visits dropped) must be caught.  The GPU kernel
+itself is compared with the oracle by the GPU parity tests.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from collision_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL_SRC = os.path.join(ROOT, "tests", "cpp", "strain_pipeline_model.c")
+
+
+def write_topology(sc, path):
+    fixed = sc.fixed.astype(bool)
+    ne = sc.nt + sc.nb
+    ev = np.full((ne, 3), -1, np.int32)
+    ev[:sc.nt] = sc.tris
+    ev[sc.nt:, :2] = sc.bonds
+    rigid = np.zeros(ne, np.int32)
+    rigid[:sc.nt] = fixed[sc.tris].any(axis=1)
+    rigid[sc.nt:] = fixed[sc.bonds].any(axis=1)
+    l0 = np.zeros((ne, 3))
+    l0[:sc.nt] = sc.tri_len0
+    l0[sc.nt:, 0] = sc.bond_len0
+    with open(path, "wb") as f:
+        np.array([sc.nv, ne], np.int32).tofile(f)
+        ev.tofile(f)
+        rigid.tofile(f)
+        sc.x.astype(np.float64).tofile(f)
+        l0.tofile(f)
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    d = tmp_path_factory.mktemp("strain_model")
+    exe = d / "model"
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-o", str(exe), MODEL_SRC, "-lm"],
+                   check=True)
+    return exe, d
+
+
+def run(model, sc_name, sc, *args):
+    exe, d = model
+    topo = d / f"{sc_name}.bin"
+    if not topo.exists():
+        write_topology(sc, topo)
+    p = subprocess.run([str(exe), str(topo)] + [str(a) for a in args], capture_output=True,
+                       text=True, timeout=120)
+    return p.returncode, p.stdout
+
+
+MESHES = {
+    "cloth_strings_boxes": lambda: scenes.combined_scene(n=40, nstrings=12),
+    "cloth_over_sphere": lambda: scenes.config0(),
+    "string_on_box": lambda: scenes.deck_string_on_static_box(),
+}
+
+# (x noise, v noise, max sweeps, level cap, seed): heavy strain (all ten
+# sweeps), mild strain (the loop stops early), none, more sweeps than one
+# pipelined launch holds, and a small level cap (split levels, larger delta)
+CASES = [
+    (0.004, 300.0, 10, 1024, 1),
+    (0.0001, 3.0, 10, 1024, 2),
+    (0.0, 0.0, 10, 1024, 3),
+    (0.004, 300.0, 23, 1024, 4),
+    (0.004, 300.0, 10, 16, 5),
+]
+
+
+@pytest.mark.parametrize("mesh", sorted(MESHES))
+@pytest.mark.parametrize("case", CASES)
+def test_pipelined_sweeps_equal_sequential_loop(model, mesh, case):
+    rc, out = run(model, mesh, MESHES[mesh](), *case)
+    assert rc == 0, out
+    assert "vertices differing=0" in out, out
+
+
+@pytest.mark.parametrize("mistake", [("-1", "0"), ("0", "1")])
+def test_planted_mistakes_are_caught(model, mistake):
+    lag, kind = mistake
+    rc, out = run(model, "cloth_strings_boxes", MESHES["cloth_strings_boxes"](),
+                  0.004, 300.0, 10, 1024, 1, lag, kind)
+    assert rc != 0, out
