@@ -576,6 +576,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   const size_t nslot = (size_t)std::max(nsched, 1);
   for (int e = 0; e < ne; ++e)
     if (level[e] >= 0) lptr[level[e] + 1]++;
+  if (nlev >= (1 << 21)) return fail(h, CS_ERR_UNSUPPORTED, "strain schedule above 2^21 levels");
   for (int l = 0; l < nlev; ++l) {
     if (lptr[l + 1] > STRAIN_LEVEL_CAP)
       return fail(h, CS_ERR_INTERNAL, "strain schedule level above its size cap");
@@ -614,7 +615,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
         if (last[v] >= 0) {
           StrainSlot& p = sslot[last[v] >> 2];
           p.ns[last[v] & 3] = 4 * s + i;
-          p.nl[last[v] & 3] = level[e];
+          p.nl[last[v] & 3] = (level[e] << 10) | (s - lptr[level[e]]);
         } else {
           first[v] = 4 * s + i;
         }
@@ -626,7 +627,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
       if (last[v] < 0) continue;
       StrainSlot& p = sslot[last[v] >> 2];
       p.ns[last[v] & 3] = first[v];
-      p.nl[last[v] & 3] = sslot[first[v] >> 2].level;
+      const int fs = first[v] >> 2, fl = sslot[fs].level;
+      p.nl[last[v] & 3] = (fl << 10) | (fs - lptr[fl]);
     }
   }
 

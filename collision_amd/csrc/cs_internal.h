@@ -73,7 +73,7 @@ struct alignas(16) StrainSlot {
   int v[3];        // vertex ids; v[2] = -1 for a bond
   int ns[3];       // 4 * slot + position of the vertex's next visit; the last
                    // visit links to the first (a single visit to itself); -1: none
-  int nl[3];       // level of that visit
+  int nl[3];       // that visit's level << 10 | its index within the level
   int level;
   dvec4 vel[3];    // velocity copies; w holds the bits of the slot that last wrote the vertex
 };

@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(SP_THREADS)
           continue;
         }
         const int ns = r.ns[p];
-        const int ts = ns >> 2, tp = ns & 3, tl = r.nl[p];
+        const int ts = ns >> 2, tp = ns & 3, tl = r.nl[p] >> 10, ti = r.nl[p] & 1023;
         if (ns < 0 || ts == s) continue;
         const int origin = wrote ? s : origin_of(r.vel[p]);
         if (!wrote && ts == origin) continue;  // back at the writer: its copy is current
@@ -820,7 +820,6 @@ __global__ void __launch_bounds__(SP_THREADS)
           spre_b[4 * (size_t)ts + tp] = 1;
           if (use_preb) bit_set(preb, tl);
         } else {
-          const int ti = ts - lvp(tl);
           atomicOr(&ring[tl & (SP_RING - 1)][ti >> 2], (1u << tp) << (8 * (ti & 3)));
           marked = true;
         }
