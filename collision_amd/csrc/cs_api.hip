@@ -669,7 +669,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.dirty_e, ne));
   HIPCHK(h, dalloc(&S.dlist, ne));
   HIPCHK(h, dalloc(&S.dpre, 2 * ne - 1));
-  HIPCHK(h, dalloc(&S.rflag, (2 * ne - 1 + 63) / 64)); HIPCHK(h, dalloc(&S.rlist, (2 * ne - 1 + 63) / 64));
+  HIPCHK(h, dalloc(&S.rflag, (2 * ne - 1 + QW_RANGE - 1) / QW_RANGE));
+  HIPCHK(h, dalloc(&S.rlist, (2 * ne - 1 + QW_RANGE - 1) / QW_RANGE));
   HIPCHK(h, dalloc(&S.zmark, nv)); HIPCHK(h, dalloc(&S.zlist, nv));
   HIPCHK(h, dalloc(&S.zkey64, nv)); HIPCHK(h, dalloc(&S.zkey64_tmp, nv));
   HIPCHK(h, dalloc(&S.zkey, nv));
@@ -697,7 +698,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemset(S.has, 0, nv));
   HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
   HIPCHK(h, hipMemset(S.dpre, 0, 2 * (size_t)ne - 1));
-  HIPCHK(h, hipMemset(S.rflag, 0, sizeof(int32_t) * ((2 * (size_t)ne - 1 + 63) / 64)));
+  HIPCHK(h, hipMemset(S.rflag, 0, sizeof(int32_t) * ((2 * (size_t)ne - 1 + QW_RANGE - 1) / QW_RANGE)));
   h->assembled = true;
   h->state_set = false;
   return CS_OK;

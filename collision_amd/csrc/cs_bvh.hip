@@ -481,10 +481,11 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
     const int4 t = ev[e];
     d = dirty_v[t.x] | dirty_v[t.y] | ((t.z >= 0) ? dirty_v[t.z] : 0);
     dirty_e[e] = d ? 1 : 0;
-    if (d) {  // the leaf, and its 64-node range of the pre-order layout
+    if (d) {  // the leaf, and its QW_RANGE-node range of the pre-order layout
       const int p = enode[e];
       dpre[p] = 1;
-      if (atomicExch(&rflag[p >> 6], 1) == 0) rlist[atomicAdd(&counters[CNT_DRANGES], 1)] = p >> 6;
+      const int r = p >> QW_RANGE_SHIFT;
+      if (atomicExch(&rflag[r], 1) == 0) rlist[atomicAdd(&counters[CNT_DRANGES], 1)] = r;
     }
   }
   const int slot = block_append(d, &counters[CNT_DIRTY], &s_cnt, &s_base);
@@ -581,7 +582,7 @@ __global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebo
 // only (previous boxes, into CNT_OLD64); emit = 1: count into CNT_NEW64 and
 // append non-rejected pairs to qpairs.
 //
-// One wavefront per 64-node range of the pre-order layout that holds dirty
+// One wavefront per QW_RANGE-node range of the pre-order layout that holds dirty
 // leaves: the range's dirty leaves are neighbours in Morton order, so the
 // union of their boxes is small.  The wavefront descends the tree breadth
 // first with that union box, one level per step with every lane loading one
@@ -637,8 +638,8 @@ __global__ void __launch_bounds__(64) k_query_ranges(
   // ranges in block order: the first nr blocks are busy, and the hardware's
   // round-robin placement spreads them over all XCDs
   for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
-    const int si = 64 * rlist[ri] + lane;  // this lane's node
-    const bool act = si < npre && dpre[si];
+    const int si = QW_RANGE * rlist[ri] + lane;  // this lane's node
+    const bool act = lane < QW_RANGE && si < npre && dpre[si];
     bvh_node me{};
     dbox mb{};
     int4 mt{};
@@ -819,8 +820,8 @@ __global__ void __launch_bounds__(64) k_clear_ranges(const int32_t* __restrict__
   const int nr = counters[CNT_DRANGES];
   for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
     const int r = rlist[ri];
-    const int p = 64 * r + (int)threadIdx.x;
-    if (p < npre) dpre[p] = 0;
+    const int p = QW_RANGE * r + (int)threadIdx.x;
+    if ((int)threadIdx.x < QW_RANGE && p < npre) dpre[p] = 0;
     if (threadIdx.x == 0) rflag[r] = 0;
   }
 }

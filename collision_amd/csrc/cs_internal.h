@@ -115,7 +115,7 @@ struct DevState {
   uint32_t* scan_part = nullptr; // partial sums of the device scan
   size_t scan_part_words = 0;
   int32_t* leaf_of = nullptr;    // element -> pre-order index of its leaf
-  // dirty leaves by pre-order index, and the 64-node ranges holding them
+  // dirty leaves by pre-order index, and the QW_RANGE-node ranges holding them
   uint8_t* dpre = nullptr;
   int32_t *rflag = nullptr, *rlist = nullptr;
   // candidate stream (current and previous pass)
@@ -184,6 +184,8 @@ void launch_traverse(hipStream_t st, DevState& S, int cap);
 // incremental passes
 void launch_mark_dirty(hipStream_t st, DevState& S);
 void launch_clear_ranges(hipStream_t st, DevState& S);
+constexpr int QW_RANGE_SHIFT = 3;
+constexpr int QW_RANGE = 1 << QW_RANGE_SHIFT;  // pre-order nodes per dirty-query range
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
 void launch_grow_dirty(hipStream_t st, DevState& S);
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);
