@@ -259,10 +259,15 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   const PrevPass prev = h->prev;
   h->prev.valid = false;
   mark(h, &T.e0);
-  HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * 4, h->st));            // pairs, hits, hp, sub
-  HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
-  HIPCHK(h, hipMemsetAsync(S.counters + CNT_DIRTY, 0, sizeof(int), h->st));
-  HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 6 * sizeof(int), h->st));
+  // the dirty-range marks of the previous pass are cleared first (their count
+  // is zeroed below)
+  if (S.ne >= 2) launch_clear_ranges(h->st, S);
+  // pairs, hits, hit pairs, sub-tests, query pairs, dirty elements, the
+  // 64-bit candidate counts and the dirty-range count
+  launch_zero_counters(h->st, S,
+                       (1u << CNT_PAIRS) | (1u << CNT_HITS) | (1u << CNT_HIT_PAIRS) |
+                           (1u << CNT_HIT_SUB) | (1u << CNT_QPAIRS) | (1u << CNT_DIRTY) |
+                           (0x3fu << CNT_CAND64) | (1u << CNT_DRANGES));
   NarrowParams NP;
   NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
   NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
@@ -321,7 +326,6 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // phase appends to them, and overflows are detected afterwards (the
     // narrow phase only appends records and performs idempotent zone
     // merges, so it can be run again).
-    launch_clear_ranges(h->st, S);
     launch_mark_dirty(h->st, S);
     HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
     mark(h, &T.t0);

@@ -912,7 +912,16 @@ void launch_clear_ranges(hipStream_t st, DevState& S) {
   const int npre = 2 * S.ne - 1;
   hipLaunchKernelGGL(k_clear_ranges, dim3(std::min((npre + 63) / 64, 2048)), dim3(64), 0, st,
                      S.rlist, npre, S.dpre, S.rflag, S.counters);
-  (void)hipMemsetAsync(S.counters + CNT_DRANGES, 0, sizeof(int), st);
+}
+
+// zero the counters whose bits are set in mask (one launch for several)
+__global__ void k_zero_counters(int* __restrict__ counters, uint32_t mask) {
+  const int i = threadIdx.x;
+  if (i < 32 && ((mask >> i) & 1u)) counters[i] = 0;
+}
+
+void launch_zero_counters(hipStream_t st, DevState& S, uint32_t mask) {
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(32), 0, st, S.counters, mask);
 }
 
 void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {

@@ -183,7 +183,10 @@ hipError_t refit_bvh(hipStream_t st, DevState& S);
 void launch_traverse(hipStream_t st, DevState& S, int cap);
 // incremental passes
 void launch_mark_dirty(hipStream_t st, DevState& S);
+// clears the previous pass's dirty-range marks; its range count is zeroed by
+// the caller afterwards (launch_zero_counters)
 void launch_clear_ranges(hipStream_t st, DevState& S);
+void launch_zero_counters(hipStream_t st, DevState& S, uint32_t mask);
 constexpr int QW_RANGE_SHIFT = 3;
 constexpr int QW_RANGE = 1 << QW_RANGE_SHIFT;  // pre-order nodes per dirty-query range
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
