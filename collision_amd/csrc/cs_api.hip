@@ -98,7 +98,7 @@ void free_dev(DevState& S) {
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
   dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist);
-  dfree(S.dpre); dfree(S.rflag); dfree(S.rlist);
+  dfree(S.dpre); dfree(S.rflag); dfree(S.rlist); dfree(S.tnext);
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
   dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
@@ -673,6 +673,13 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.dirty_e, ne));
   HIPCHK(h, dalloc(&S.dlist, ne));
   HIPCHK(h, dalloc(&S.dpre, 2 * ne - 1));
+  HIPCHK(h, dalloc(&S.tnext, kNumXcd));
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->P.device) == hipSuccess &&
+        cus > 0)
+      S.num_cus = cus;
+  }
   HIPCHK(h, dalloc(&S.rflag, (2 * ne - 1 + QW_RANGE - 1) / QW_RANGE));
   HIPCHK(h, dalloc(&S.rlist, (2 * ne - 1 + QW_RANGE - 1) / QW_RANGE));
   HIPCHK(h, dalloc(&S.zmark, nv)); HIPCHK(h, dalloc(&S.zlist, nv));

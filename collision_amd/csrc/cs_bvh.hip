@@ -376,6 +376,17 @@ __device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
 constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flush
 
+// Persistent full-pass self query.  Lane = one leaf's stackless walk in
+// depth-first order after the leaf (every leaf j > i, each subtree once).
+// Walk lengths differ a lot within a wavefront, so lanes whose walk has
+// ended take the next leaves (a wavefront-aggregated atomic on a per-XCD
+// counter) while the others go on; each XCD starts on its own contiguous
+// eighth of the leaves (L2 locality) and then helps the others.  Pairs are
+// buffered per lane and written by the whole wavefront when a buffer fills.
+constexpr int PT_BLOCKS_PER_CU = 8;
+constexpr int PT_REFILL = 16;   // refill once this many lanes are idle
+constexpr int PT_STEPS = 16;    // walk steps between refill checks
+
 __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
                                                            const int32_t* __restrict__ lpre,
                                                            const dbox* __restrict__ ebox,
@@ -383,26 +394,84 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
                                                            const uint8_t* __restrict__ eflags,
                                                            int n, int2* __restrict__ pairs, int cap,
                                                            int* __restrict__ pair_count,
-                                                           unsigned long long* __restrict__ cand_count) {
+                                                           unsigned long long* __restrict__ cand_count,
+                                                           int* __restrict__ next) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
-  __shared__ int s_off[TRAV_THREADS];
-  __shared__ unsigned long long s_cand[TRAV_THREADS / 64];
-  __shared__ int s_base;
-  const int t = threadIdx.x;
-  const int i = xcd_block(blockIdx.x, gridDim.x) * TRAV_THREADS + t;
+  const int t = threadIdx.x, lane = t & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  // the eighth of the leaves this block's XCD starts on (blocks are dealt
+  // to the XCDs round-robin)
+  int chunk = (int)(blockIdx.x % kNumXcd), tried = 0;
+  auto chunk_end = [&](int c) { return (int)(((int64_t)n * (c + 1)) / kNumXcd); };
   unsigned long long cand = 0;
   int nbuf = 0;
-  if (i < n && n > 1) {
-    const bvh_node me = nodes[lpre[i]];
-    const int e = me.left;
-    const dbox mb = ebox[e];
-    const int4 mt = ev[e];
-    const uint8_t mf = eflags[e];
-    // depth-first order after leaf i: every leaf j > i, each subtree once
-    int nd = me.rope;
-    while (nd >= 0) {
+  int nd = -1;  // current node of this lane's walk, -1: idle
+  int e = 0;
+  float mlo[3] = {0.f, 0.f, 0.f}, mhi[3] = {0.f, 0.f, 0.f};
+  dbox mb{};
+  int4 mt{};
+  uint8_t mf = 0;
+  auto flush = [&]() {  // the wavefront's buffered pairs
+    int incl = nbuf;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    const int total = __shfl(incl, 63);
+    if (total == 0) return;
+    int base = 0;
+    if (lane == 63) base = atomicAdd(pair_count, total);
+    base = __shfl(base, 63) + incl - nbuf;
+    for (int k = 0; k < nbuf; ++k)
+      if (base + k < cap) pairs[base + k] = buf[k * TRAV_THREADS + t];
+    nbuf = 0;
+  };
+  bool more = n > 1;  // leaves left to take (uniform)
+  for (;;) {
+    // refill idle lanes
+    const uint64_t idle = __ballot(nd < 0);
+    if (more && (__popcll(idle) >= PT_REFILL || idle == ~0ull)) {
+      const int want = __popcll(idle);
+      int got = 0, first = 0;
+      while (got == 0 && tried < kNumXcd) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&next[chunk], want);
+        base = __shfl(base, 0);
+        const int start = (int)(((int64_t)n * chunk) / kNumXcd) + base;
+        const int end = chunk_end(chunk);
+        if (start < end) {
+          got = min(want, end - start);
+          first = start;
+        } else {
+          chunk = (chunk + 1) % kNumXcd;
+          ++tried;
+        }
+      }
+      if (got == 0) more = false;
+      if (nd < 0) {
+        const int r = __popcll(idle & below);
+        if (r < got) {
+          const int i = first + r;
+          const bvh_node me = nodes[lpre[i]];
+          e = me.left;
+          mb = ebox[e];
+          mt = ev[e];
+          mf = eflags[e];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            mlo[d] = me.lo[d];
+            mhi[d] = me.hi[d];
+          }
+          nd = me.rope;
+        }
+      }
+    }
+    if (!more && __ballot(nd >= 0) == 0) break;
+    // walk
+    for (int step = 0; step < PT_STEPS && nd >= 0; ++step) {
       const bvh_node node = nodes[nd];
-      if (!f32_overlap(node, me.lo, me.hi)) {
+      if (!f32_overlap(node, mlo, mhi)) {
         nd = node.rope;
         continue;
       }
@@ -418,38 +487,16 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
       const int4 ot = ev[e2];
       const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
       if (rej) continue;
-      const int2 pr = make_int2(min(e, e2), max(e, e2));
-      if (nbuf < TRAV_BUF) {
-        buf[nbuf * TRAV_THREADS + t] = pr;
-        ++nbuf;
-      } else {
-        const int slot = atomicAdd(pair_count, 1);
-        if (slot < cap) pairs[slot] = pr;
-      }
+      buf[nbuf * TRAV_THREADS + t] = make_int2(min(e, e2), max(e, e2));
+      if (++nbuf == TRAV_BUF) break;  // the buffer is written before the walk goes on
     }
+    if (__ballot(nbuf == TRAV_BUF)) flush();
   }
-  // block-wide exclusive scan of the buffered counts, one atomic per block
-  s_off[t] = nbuf;
-  for (int off = 32; off > 0; off >>= 1) cand += __shfl_down(cand, off);
-  if ((t & 63) == 0) s_cand[t >> 6] = cand;
-  __syncthreads();
-  for (int off = 1; off < TRAV_THREADS; off <<= 1) {
-    const int add = (t >= off) ? s_off[t - off] : 0;
-    __syncthreads();
-    s_off[t] += add;
-    __syncthreads();
-  }
-  if (t == TRAV_THREADS - 1) {
-    const int total = s_off[t];
-    s_base = total ? atomicAdd(pair_count, total) : 0;
-    unsigned long long c = 0;
-    for (int w = 0; w < TRAV_THREADS / 64; ++w) c += s_cand[w];
-    if (c) atomicAdd(cand_count, c);
-  }
-  __syncthreads();
-  const int base = s_base + s_off[t] - nbuf;
-  for (int k = 0; k < nbuf; ++k)
-    if (base + k < cap) pairs[base + k] = buf[k * TRAV_THREADS + t];
+  flush();
+  unsigned long long c = cand;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0 && c) atomicAdd(cand_count, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,10 +1076,12 @@ hipError_t refit_bvh(hipStream_t st, DevState& S) {
 
 void launch_traverse(hipStream_t st, DevState& S, int cap) {
   const int n = S.ne;
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_traverse, dim3((n + TRAV_THREADS - 1) / TRAV_THREADS), dim3(TRAV_THREADS), 0,
-                     st, S.nodes, S.lpre, S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
-                     (unsigned long long*)(S.counters + CNT_CAND64));
+  if (n < 2) return;
+  (void)hipMemsetAsync(S.tnext, 0, sizeof(int) * kNumXcd, st);
+  const int blocks = std::min(S.num_cus * PT_BLOCKS_PER_CU, (n + TRAV_THREADS - 1) / TRAV_THREADS);
+  hipLaunchKernelGGL(k_traverse, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st, S.nodes, S.lpre,
+                     S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
+                     (unsigned long long*)(S.counters + CNT_CAND64), S.tnext);
 }
 
 }  // namespace cs

@@ -118,6 +118,8 @@ struct DevState {
   // dirty leaves by pre-order index, and the QW_RANGE-node ranges holding them
   uint8_t* dpre = nullptr;
   int32_t *rflag = nullptr, *rlist = nullptr;
+  int32_t* tnext = nullptr;        // persistent full traversal: next leaf per XCD share
+  int num_cus = 256;               // compute units of the device
   // candidate stream (current and previous pass)
   int2* pairs = nullptr;
   int2* pairs_prev = nullptr;
