@@ -481,11 +481,14 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
       }
       nd = node.rope;
       const int e2 = node.left;
+      // box, topology and flags in one round trip (the exact overlap test
+      // almost always passes where the rounded-out tree box did)
       const dbox ob = ebox[e2];
+      const int4 ot = ev[e2];
+      const uint8_t of = eflags[e2];
       if (!f64_overlap(ob, mb)) continue;
       ++cand;
-      const int4 ot = ev[e2];
-      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, eflags[e2]);
+      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, of);
       if (rej) continue;
       buf[nbuf * TRAV_THREADS + t] = make_int2(min(e, e2), max(e, e2));
       if (++nbuf == TRAV_BUF) break;  // the buffer is written before the walk goes on
