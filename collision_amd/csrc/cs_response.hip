@@ -496,8 +496,9 @@ __device__ __attribute__((always_inline)) inline bool edge_over_limit_l(const do
 // nothing was over the limit.  Changes travel along each vertex's cycle of
 // visits as a token: the visit that writes a vertex stores the new value
 // into the next visit's velocity copy and marks it; a marked visit that does
-// not write passes the value on unless the next visit is the writer itself
-// (whose copy already holds the value).  A sweep in which nothing is over the
+// not write passes the value on.  The token ends where it started: the
+// writer receives the value (refreshing its own copy) but is not marked, as
+// it is evaluated anyway because it wrote.  A sweep in which nothing is over the
 // limit ends the loop; it leaves no token and no written slot behind, so the
 // sweeps after it evaluate nothing.
 //
@@ -803,19 +804,22 @@ __global__ void __launch_bounds__(SP_THREADS)
       for (int p = 0; p < 3; ++p) {
         if (p >= np) break;
         const bool wrote = (wmask >> p) & 1u;
-        if (wrote) {
-          slots[s].vel[p] = vel_with_origin(a[p], s);
-        } else if (!((f >> p) & 1u)) {
+        if (!wrote && !((f >> p) & 1u)) continue;
+        const int ns = r.ns[p];
+        if (ns < 0) continue;
+        const int ts = ns >> 2, tp = ns & 3, tl = r.nl[p] >> 10, ti = r.nl[p] & 1023;
+        const bool wrap = tl < l;  // the vertex's first visit, next sweep
+        // no delivery comes back to this slot (a single visit, or the last
+        // visit in the last sweep in flight): it keeps the value itself
+        if (ts == s || (wrap && k == K - 1)) {
+          if (wrote) slots[s].vel[p] = vel_with_origin(a[p], s);
           continue;
         }
-        const int ns = r.ns[p];
-        const int ts = ns >> 2, tp = ns & 3, tl = r.nl[p] >> 10, ti = r.nl[p] & 1023;
-        if (ns < 0 || ts == s) continue;
         const int origin = wrote ? s : origin_of(r.vel[p]);
-        if (!wrote && ts == origin) continue;  // back at the writer: its copy is current
-        const bool wrap = tl < l;               // first visit, next sweep
-        if (wrap && k == K - 1) continue;
         slots[ts].vel[tp] = vel_with_origin(a[p], origin);
+        // back at the writer: delivered, not marked (it is evaluated anyway,
+        // because it wrote)
+        if (ts == origin) continue;
         if (!wrap && tl - l >= SP_RING) {       // far ahead (one sweep in flight)
           spre_b[4 * (size_t)ts + tp] = 1;
           if (use_preb) bit_set(preb, tl);

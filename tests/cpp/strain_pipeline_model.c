@@ -138,16 +138,22 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
         d4 a[3]; for(int p=0;p<np;p++) a[p]=cp[s][p];
         int ed=0, dg=0; int w=eval_elem(e,a,&ed,&dg);
         edges[k]+=ed; *degen_sum+=dg;
-        for(int p=0;p<np;p++) if(w>>p&1){ d4 c=a[p]; c.w=s; cp[s][p]=c; }
+        /* A changed vertex's value goes to its next visit, with the id of
+           the slot that wrote it.  The next visit is marked (evaluated)
+           unless it is that writer, which is evaluated anyway because it
+           wrote; the writer's own copy is refreshed by that delivery, so
+           it only stores its own copy when no delivery will come: a vertex
+           with one visit, or the vertex's last visit in the last sweep. */
         for(int p=0;p<np;p++){
           double origin;
-          if (w>>p&1) origin=s; else if ((f>>p&1) && mistake!=1){ origin=cp[s][p].w; } else continue;
+          const int wrote = w>>p&1;
+          if (wrote) origin=s; else if ((f>>p&1) && mistake!=1){ origin=cp[s][p].w; } else continue;
           int ns=nslot[s][p]; int ts=ns>>2, tpos=ns&3, tl=nlev[s][p];
-          if (ts==s) continue;
-          if (!(w>>p&1) && ts==(int)origin) continue;
-          int wrap = tl < l; if (wrap && k==K-1) continue;
+          int wrap = tl < l;
+          if (ts==s || (wrap && k==K-1)) { if (wrote) { d4 c=a[p]; c.w=s; cp[s][p]=c; } continue; }
           if (!wrap && tl - l >= R){ printf("far mark unsupported in model\n"); exit(1);} 
           d4 c=a[p]; c.w=origin; cp[ts][tpos]=c;
+          if (ts==(int)origin) continue;
           int ri=tl&(R-1); if(owner[ri]!=-1 && owner[ri]!=tl){ printf("ring alias mark t=%d tl=%d owner=%d\n",t,tl,owner[ri]); exit(1);} owner[ri]=tl;
           ring[ri][ts-lp[tl]] |= 1<<tpos;
         }
