@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(256) k_zone_heads(const uint64_t* __restrict__
 // members l, l+64, ... (ascending vertex id) from 0.0, then the 64 lane sums
 // are added in lane order.
 constexpr int ZONE_LANES = 64;
+constexpr int ZU = 2;  // member loads in flight per lane
 
 __device__ inline void lane_total(double (*part)[9], int nout, double* tot) {
   for (int k = 0; k < nout; ++k) tot[k] = 0.0;
@@ -217,13 +218,26 @@ __global__ void __launch_bounds__(ZONE_LANES)
     const int e = zend[zkey[b]];
     const int n = e - b;
     // x_cm, v_cm
+    // (member loads ZU at a time, summed in member order)
     double t[9];
     for (int k = 0; k < 6; ++k) t[k] = 0.0;
-    for (int j = b + l; j < e; j += ZONE_LANES) {
-      const int p = zval[j];
-      const dvec4 o = x_old[p], a = av[p];
-      t[0] = o.x + t[0]; t[1] = o.y + t[1]; t[2] = o.z + t[2];
-      t[3] = a.x + t[3]; t[4] = a.y + t[4]; t[5] = a.z + t[5];
+    for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
+      dvec4 o[ZU], a[ZU];
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        const int j = j0 + u * ZONE_LANES;
+        if (j < e) {
+          const int p = zval[j];
+          o[u] = x_old[p];
+          a[u] = av[p];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        if (j0 + u * ZONE_LANES >= e) break;
+        t[0] = o[u].x + t[0]; t[1] = o[u].y + t[1]; t[2] = o[u].z + t[2];
+        t[3] = a[u].x + t[3]; t[4] = a[u].y + t[4]; t[5] = a[u].z + t[5];
+      }
     }
     for (int k = 0; k < 6; ++k) part[l][k] = t[k];
     __syncthreads();
@@ -234,15 +248,27 @@ __global__ void __launch_bounds__(ZONE_LANES)
     __syncthreads();
     // angular momentum L = sum m dx x dv
     for (int k = 0; k < 3; ++k) t[k] = 0.0;
-    for (int j = b + l; j < e; j += ZONE_LANES) {
-      const int p = zval[j];
-      const dvec4 o = x_old[p], a = av[p];
-      const double xo[3] = {o.x, o.y, o.z}, vv[3] = {a.x, a.y, a.z};
-      double dx[3], dv[3], Li[3];
-      sub3(xo, x_cm, dx);
-      sub3(vv, v_cm, dv);
-      cross3(dx, dv, Li);
-      for (int k = 0; k < 3; ++k) t[k] = m * Li[k] + t[k];
+    for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
+      dvec4 o[ZU], a[ZU];
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        const int j = j0 + u * ZONE_LANES;
+        if (j < e) {
+          const int p = zval[j];
+          o[u] = x_old[p];
+          a[u] = av[p];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        if (j0 + u * ZONE_LANES >= e) break;
+        const double xo[3] = {o[u].x, o[u].y, o[u].z}, vv[3] = {a[u].x, a[u].y, a[u].z};
+        double dx[3], dv[3], Li[3];
+        sub3(xo, x_cm, dx);
+        sub3(vv, v_cm, dv);
+        cross3(dx, dv, Li);
+        for (int k = 0; k < 3; ++k) t[k] = m * Li[k] + t[k];
+      }
     }
     for (int k = 0; k < 3; ++k) part[l][k] = t[k];
     __syncthreads();
@@ -251,19 +277,27 @@ __global__ void __launch_bounds__(ZONE_LANES)
     __syncthreads();
     // inertia tensor I = sum m (|dx|^2 delta - dx dx^T)
     for (int k = 0; k < 9; ++k) t[k] = 0.0;
-    for (int j = b + l; j < e; j += ZONE_LANES) {
-      const int p = zval[j];
-      const dvec4 o = x_old[p];
-      const double xo[3] = {o.x, o.y, o.z};
-      double dx[3];
-      sub3(xo, x_cm, dx);
-      const double mag_dx = mag3(dx);
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          double tm = -dx[r] * dx[c];
-          if (r == c) tm += mag_dx * mag_dx;
-          t[3 * r + c] = tm * m + t[3 * r + c];
-        }
+    for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
+      dvec4 o[ZU];
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        const int j = j0 + u * ZONE_LANES;
+        if (j < e) o[u] = x_old[zval[j]];
+      }
+#pragma unroll
+      for (int u = 0; u < ZU; ++u) {
+        if (j0 + u * ZONE_LANES >= e) break;
+        const double xo[3] = {o[u].x, o[u].y, o[u].z};
+        double dx[3];
+        sub3(xo, x_cm, dx);
+        const double mag_dx = mag3(dx);
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            double tm = -dx[r] * dx[c];
+            if (r == c) tm += mag_dx * mag_dx;
+            t[3 * r + c] = tm * m + t[3 * r + c];
+          }
+      }
     }
     for (int k = 0; k < 9; ++k) part[l][k] = t[k];
     __syncthreads();
@@ -282,6 +316,9 @@ __global__ void __launch_bounds__(ZONE_LANES)
       else w[c] = det3(tmp) / det3(I);
     }
     const double mag_w = mag3(w);
+    // the rotation of the zone, the same for every member
+    const double cw = cs_det_cos(dt * mag_w);
+    const double sw = mag_w < CS_ROUND_EPS ? 0.0 : cs_det_sin(dt * mag_w) / mag_w;
     for (int j = b + l; j < e; j += ZONE_LANES) {
       const int p = zval[j];
       if (fixed[p]) continue;
@@ -299,12 +336,11 @@ __global__ void __launch_bounds__(ZONE_LANES)
         const double sc = dot3(dx, w) / dot3(w, w);
         for (int k = 0; k < 3; ++k) xF[k] = sc * w[k];
         sub3(dx, xF, xR);
-        const double sw = cs_det_sin(dt * mag_w) / mag_w;
         for (int k = 0; k < 3; ++k) tmpV[k] = sw * w[k];
         cross3(tmpV, xR, wxR);
       }
       for (int k = 0; k < 3; ++k) {
-        x_new[k] = x_cm[k] + dt * v_cm[k] + xF[k] + cs_det_cos(dt * mag_w) * xR[k] + wxR[k];
+        x_new[k] = x_cm[k] + dt * v_cm[k] + xF[k] + cw * xR[k] + wxR[k];
         an[k] = (x_new[k] - xo[k]) / dt;
       }
       if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
