@@ -599,7 +599,7 @@ __global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebo
                                                     const int* __restrict__ counters,
                                                     const int32_t* __restrict__ enode,
                                                     bvh_node* nodes,
-                                                    const int32_t* __restrict__ parent, int n) {
+                                                    const int32_t* __restrict__ parent) {
   const int nd = counters[CNT_DIRTY];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
     const int e = dlist[k];
@@ -765,9 +765,8 @@ __global__ void __launch_bounds__(64) k_query_ranges(
       nfr = nnext;
     }
     int nbuf = 0;
-    // a box-overlapping pair (query element qel at pre-order qsi, leaf j)
-    auto take = [&](int qsi, int qel, const int4& qtop, uint8_t qfl, int j, int e2,
-                    const int4& ot, uint8_t of) {
+    // a box-overlapping pair of query element qel and element e2
+    auto take = [&](int qel, const int4& qtop, uint8_t qfl, int e2, const int4& ot, uint8_t of) {
       ++cand;
       if (!emit) return;
       const bool rej = (qel > e2) ? pair_rejected(qtop, ot, qfl) : pair_rejected(ot, qtop, of);
@@ -780,7 +779,6 @@ __global__ void __launch_bounds__(64) k_query_ranges(
         const int slot = atomicAdd(&counters[CNT_QPAIRS], 1);
         if (slot < qcap) qpairs[slot] = pr;
       }
-      (void)qsi;
     };
     if (!overflow) {
       // lanes hold the collected leaves, 64 at a time; the queries are
@@ -827,7 +825,7 @@ __global__ void __launch_bounds__(64) k_query_ranges(
         while (hits) {
           const int q = __builtin_ctzll(hits);
           hits &= hits - 1;
-          take(qr[q].si, qe[q], qt[q], qf[q], j, e2, ot, of);
+          take(qe[q], qt[q], qf[q], e2, ot, of);
         }
       }
     } else if (act) {
@@ -850,7 +848,7 @@ __global__ void __launch_bounds__(64) k_query_ranges(
         const uint8_t od = dirty_e[e2];
         if (od && j < si) continue;  // counted from e2
         if (!f64_overlap(ebox[e2], mb)) continue;
-        take(si, e, mt, mf, j, e2, ev[e2], eflags[e2]);
+        take(e, mt, mf, e2, ev[e2], eflags[e2]);
       }
     }
     if (emit) flush_pairs(buf, nbuf, qpairs, qcap, counters);
@@ -983,7 +981,7 @@ void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
 void launch_grow_dirty(hipStream_t st, DevState& S) {
   if (S.ne < 2) return;
   hipLaunchKernelGGL(k_grow_dirty, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st,
-                     S.ebox, S.dlist, S.counters, S.leaf_of, S.nodes, S.parent, S.ne);
+                     S.ebox, S.dlist, S.counters, S.leaf_of, S.nodes, S.parent);
 }
 
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {

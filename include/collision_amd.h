@@ -106,7 +106,8 @@ typedef struct cs_step_report {
     int32_t strain_sweeps;
     int32_t strain_edges;    /* edges over the limit in the last sweep */
     int32_t degenerate_edges;
-    int32_t strain_levels;   /* schedule levels the strain sweeps evaluated (sum) */
+    int32_t strain_levels;   /* level steps the strain sweeps ran (sweeps in flight
+                                together advance one level per step; sum) */
     int64_t candidates;      /* sum over passes; -1 when a pass reports -1 */
     int64_t narrow_pairs;    /* sum over passes; -1 when a pass reports -1 */
     int64_t hit_pairs, hit_subtests;
