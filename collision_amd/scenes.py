@@ -508,6 +508,34 @@ def deck_string_on_static_box(dt=1e-3, steps=150, segs=40):
                    friction=0.3)
 
 
+def fan_disc_on_box(nfan=96, rings=3, vsig=2.0, dt=1e-3, steps=8, seed=7):
+    """A flat disc meshed as a fan of `nfan` triangles around one centre
+    vertex plus `rings` - 1 bands, with noisy velocities, falling onto a
+    static box.  The centre vertex is shared by `nfan` elements, so its
+    visits in a strain sweep span at least `nfan` levels."""
+    b = _Builder()
+    s, c = det_sincos_cycles(np.arange(nfan) / nfan)
+    pts = [np.array([[0.0, 0.0, 0.01]])]
+    for r in range(1, rings + 1):
+        pts.append(np.stack([0.02 * r * c, 0.02 * r * s, np.full(nfan, 0.01)], 1))
+    x = np.concatenate(pts)
+    tris = []
+    for i in range(nfan):
+        tris.append((0, 1 + i, 1 + (i + 1) % nfan))
+    for r in range(1, rings):
+        o0, o1 = 1 + (r - 1) * nfan, 1 + r * nfan
+        for i in range(nfan):
+            j = (i + 1) % nfan
+            tris.append((o0 + i, o1 + i, o1 + j))
+            tris.append((o0 + i, o1 + j, o0 + j))
+    v = normal(seed, x.size, vsig).reshape(-1, 3)
+    v[:, 2] -= 1.0
+    b.add_surface(x, v, np.array(tris, np.int32), fixed=False)
+    bx, bt = box_mesh((0.0, 0.0, -0.1), (0.3, 0.3, 0.1))
+    b.add_surface(bx, np.zeros_like(bx), bt, fixed=True)
+    return b.build("fan_disc_on_box", thickness=1e-3, dt=dt, steps=steps)
+
+
 CONFIGS = {
     "config0": config0,
     "config1": config1,

@@ -107,6 +107,7 @@ static void schedule(int W0, int CAP){
 /* ---- pipelined model ---- */
 #define R 64
 #define CAPL 1024
+static long far_marks = 0;
 static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* iters, long* evals){
   int nsl=lp[L];
   d4 (*cp)[3]=malloc(sizeof(d4)*3*nsl); /* local copies, w = origin */
@@ -121,6 +122,9 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
     for(int s=0;s<nsl;s++){ int e=slot_elem[s]; int np=T[e][2]>=0?3:2; d4 a[3]; for(int k=0;k<np;k++) a[k]=av[T[e][k]]; int ed=0,dg=0; eval_elem(e,a,&ed,&dg); /* note: pre-check in kernel tests each edge on the pre-sweep velocities separately */ pre[s]=0;
       { int g; const double* o[3]={XO[T[e][0]],XO[T[e][1]],np==3?XO[T[e][2]]:NULL}; d4 b[3]; for(int k=0;k<np;k++) b[k]=av[T[e][k]]; int any=over(o[0],o[1],&b[0],&b[1],L0[e][0],&g); if(np==3){ any|=over(o[1],o[2],&b[1],&b[2],L0[e][1],&g); any|=over(o[2],o[0],&b[2],&b[0],L0[e][2],&g);} pre[s]=any; anypre|=any; } }
     if (!anypre){ total_sweeps++; *last_edges=0; break; }
+    /* marks R or more levels ahead (one sweep in flight): per-slot bytes,
+       taken by the first sweep's scan of that level */
+    uint8_t* far=calloc(nsl,1);
     uint8_t ring[R][CAPL]; memset(ring,0,sizeof ring); int owner[R]; for(int i=0;i<R;i++) owner[i]=-1;
     long edges[16]={0};
     int tend = L + (K-1)*dl;
@@ -128,7 +132,7 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
       /* scan */
       int nw=0; static int wk_s[16*CAPL], wk_k[16*CAPL], wk_f[16*CAPL];
       for (int k=0;k<K;k++){ int l=t-k*dl; if(l<0||l>=L) continue; int n=lp[l+1]-lp[l];
-        for(int i=0;i<n;i++){ int f=ring[l&(R-1)][i]; if(k==0 && pre[lp[l]+i]) f|=8; if(f){ wk_s[nw]=lp[l]+i; wk_k[nw]=k; wk_f[nw]=f; nw++; } ring[l&(R-1)][i]=0; }
+        for(int i=0;i<n;i++){ int f=ring[l&(R-1)][i]; if(k==0){ if(pre[lp[l]+i]) f|=8; f|=far[lp[l]+i]; far[lp[l]+i]=0; } if(f){ wk_s[nw]=lp[l]+i; wk_k[nw]=k; wk_f[nw]=f; nw++; } ring[l&(R-1)][i]=0; }
         if (owner[l&(R-1)]!=-1 && owner[l&(R-1)]!=l){ printf("ring alias at t=%d l=%d owner=%d\n",t,l,owner[l&(R-1)]); exit(1);} owner[l&(R-1)]=-1; }
       if (nw) (*iters)++;
       *evals+=nw;
@@ -151,9 +155,9 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
           int ns=nslot[s][p]; int ts=ns>>2, tpos=ns&3, tl=nlev[s][p];
           int wrap = tl < l;
           if (ts==s || (wrap && k==K-1)) { if (wrote) { d4 c=a[p]; c.w=s; cp[s][p]=c; } continue; }
-          if (!wrap && tl - l >= R){ printf("far mark unsupported in model\n"); exit(1);} 
           d4 c=a[p]; c.w=origin; cp[ts][tpos]=c;
           if (ts==(int)origin) continue;
+          if (!wrap && tl - l >= R){ if (K!=1){ printf("far mark with %d sweeps in flight\n",K); exit(1);} far[ts] |= 1<<tpos; far_marks++; continue; }
           int ri=tl&(R-1); if(owner[ri]!=-1 && owner[ri]!=tl){ printf("ring alias mark t=%d tl=%d owner=%d\n",t,tl,owner[ri]); exit(1);} owner[ri]=tl;
           ring[ri][ts-lp[tl]] |= 1<<tpos;
         }
@@ -164,7 +168,7 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
     for(int v=0;v<nv;v++) if(la_slot[v]>=0){ d4 c=cp[la_slot[v]][la_pos[v]]; av[v].x=c.x; av[v].y=c.y; av[v].z=c.z; }
     int n=0; while(n<K && edges[n]>0) n++; int niter = n<K ? n+1 : K;
     total_sweeps += niter; *last_edges = edges[niter-1];
-    free(pre);
+    free(pre); free(far);
     if (niter < K || edges[K-1]==0) break;
     k0 += K;
     for(int s=0;s<nsl;s++){ int e=slot_elem[s]; int np=T[e][2]>=0?3:2; for(int k=0;k<np;k++){ cp[s][k]=av[T[e][k]]; cp[s][k].w=-1; } }
@@ -186,5 +190,5 @@ int main(int argc, char** argv){
   d4* a1=malloc(sizeof(d4)*nv); d4* a2=malloc(sizeof(d4)*nv); memcpy(a1,av0,sizeof(d4)*nv); memcpy(a2,av0,sizeof(d4)*nv);
   int e1,e2; long g1,g2,it,ev; int n1=sequential(a1,maxit,&e1,&g1); int n2=pipelined(a2,maxit,&e2,&g2,&it,&ev);
   long diff=0; for(int v=0;v<nv;v++) if(a1[v].x!=a2[v].x||a1[v].y!=a2[v].y||a1[v].z!=a2[v].z) diff++;
-  printf("seq: sweeps=%d edges=%d degen=%ld | pipe: sweeps=%d edges=%d degen=%ld iters=%ld evals=%ld | vertices differing=%ld\n", n1,e1,g1,n2,e2,g2,it,ev,diff);
+  printf("seq: sweeps=%d edges=%d degen=%ld | pipe: sweeps=%d edges=%d degen=%ld iters=%ld evals=%ld far=%ld | vertices differing=%ld\n", n1,e1,g1,n2,e2,g2,it,ev,far_marks,diff);
   return diff!=0 || n1!=n2 || e1!=e2 || g1!=g2; }

@@ -42,6 +42,27 @@ def test_config4_reduced_cloth_strings_boxes(full):
     assert st["zone_steps"] > 0
 
 
+def test_strain_more_sweeps_than_one_launch():
+    """The config-4 scene at 96x96 with the strain loop allowed 23 sweeps
+    instead of ten (dcollid.cpp:590): from step 26 on every step runs all 23,
+    which the device does as three launches of sweeps in flight (10, 10, 3),
+    each starting from the previous launch's velocities."""
+    sc = S.combined_scene(n=96, nstrings=60, steps=40)
+    st = run_parity(sc, steps=40, check_every=5, verbose=True, max_strain_sweeps=23)
+    assert st["strain_sweeps"] > 10 * 23
+
+
+def test_strain_fan_one_sweep_in_flight():
+    """A disc meshed as a fan of 96 triangles around one vertex: its visits
+    span so many levels that only one sweep is in flight per launch, and
+    velocity changes for visits 64 or more levels ahead are flagged in
+    global memory instead of the on-chip ring; 23 sweeps allowed."""
+    sc = S.fan_disc_on_box()
+    st = run_parity(sc, steps=sc.steps, verbose=True, max_strain_sweeps=23)
+    assert st["strain_sweeps"] > 5 * 23
+    assert st["hit_pairs"] > 0
+
+
 @pytest.mark.parametrize("maker", [S.deck_ball_ball, S.deck_string_box, S.deck_string_string,
                                    S.deck_string_on_static_box])
 def test_reference_deck_scenarios(maker):

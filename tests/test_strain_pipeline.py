@@ -5,7 +5,8 @@ The device kernel k_strain_pipe runs up to ten reduceSuperelastOnce sweeps
 a span-bounded schedule, and evaluates only the slots whose result can
 change.  tests/cpp/strain_pipeline_model.c restates that scheme on the CPU and
 compares it bit for bit with the sequential loop (dcollid.cpp:583-596) on the
-meshes below; two planted mistakes (sweeps run closer than the schedule
+meshes below (the fan disc's schedule leaves one sweep in flight and flags
+visits 64 or more levels ahead outside the ring); two planted mistakes (sweeps run closer than the schedule
 allows; tokens of non-writing visits dropped) must be caught.  The GPU kernel
 itself is compared with the oracle by the GPU parity tests.
 """
@@ -64,6 +65,7 @@ MESHES = {
     "cloth_strings_boxes": lambda: scenes.combined_scene(n=40, nstrings=12),
     "cloth_over_sphere": lambda: scenes.config0(),
     "string_on_box": lambda: scenes.deck_string_on_static_box(),
+    "fan_disc": lambda: scenes.fan_disc_on_box(),
 }
 
 # (x noise, v noise, max sweeps, level cap, seed): heavy strain (all ten
@@ -84,6 +86,10 @@ def test_pipelined_sweeps_equal_sequential_loop(model, mesh, case):
     rc, out = run(model, mesh, MESHES[mesh](), *case)
     assert rc == 0, out
     assert "vertices differing=0" in out, out
+    if mesh == "fan_disc" and case[1] > 0:
+        # one sweep in flight; visits 64 or more levels ahead are flagged
+        # outside the ring
+        assert "far=0 " not in out, out
 
 
 @pytest.mark.parametrize("mistake", [("-1", "0"), ("0", "1")])
