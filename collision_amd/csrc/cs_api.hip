@@ -32,6 +32,15 @@ struct PrevPass {
   bool valid = false;  // previous pass was a CCD / zone pass of this step
   int npairs = 0, nhits = 0, nhp = 0;
   int nq = 0;  // re-queried pairs (incremental pass)
+  int nzone = 0;  // zone members whose velocities the pass rewrote (zlist prefix)
+  unsigned long long cand = 0;
+};
+
+// the swept-box candidate pairs the proximity pass found for the first CCD
+// pass (S.pairs_seed), under the velocities before the proximity pass's hits
+struct SeedPairs {
+  bool valid = false;
+  int npairs = 0;
   unsigned long long cand = 0;
 };
 
@@ -56,6 +65,7 @@ struct cs_solver {
   std::vector<hipEvent_t> events;
   size_t ev_used = 0;
   PrevPass prev;
+  SeedPairs seed;
 };
 
 namespace {
@@ -97,7 +107,8 @@ void free_dev(DevState& S) {
   dfree(S.rmin); dfree(S.rmax); dfree(S.arrive); dfree(S.pairs); dfree(S.hits);
   dfree(S.top_start); dfree(S.scan_part); dfree(S.leaf_of);
   dfree(S.pairs_prev); dfree(S.qpairs); dfree(S.hits_prev); dfree(S.hp); dfree(S.hp_prev);
-  dfree(S.dirty_v); dfree(S.dirty_e); dfree(S.dlist);
+  dfree(S.dirty_e); dfree(S.dlist); dfree(S.pbox); dfree(S.pairs_seed);
+  dfree(S.vlist); dfree(S.vadj_ptr); dfree(S.vadj);
   dfree(S.dpre); dfree(S.rflag); dfree(S.rlist); dfree(S.tnext);
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
@@ -250,6 +261,12 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   const cs_params& P = h->P;
   const int ccd = kind != CS_PASS_PROXIMITY;
   const bool incremental = ccd && h->prev.valid && !P.force_full_passes;
+  // the first CCD pass starts from the proximity pass's swept-box pairs
+  const bool seeded = kind == CS_PASS_CCD && !incremental && h->seed.valid;
+  // the proximity pass walks the tree once for itself and the first CCD pass
+  const bool dual = kind == CS_PASS_PROXIMITY && !P.force_full_passes;
+  const SeedPairs seed = h->seed;
+  h->seed.valid = false;
   const bool exact_counts = P.log_pairs != 0;
   PassTimes T{};
   // the previous pass's results become "prev"; this pass fills the others
@@ -264,10 +281,15 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   if (S.ne >= 2) launch_clear_ranges(h->st, S);
   // pairs, hits, hit pairs, sub-tests, query pairs, dirty elements, the
   // 64-bit candidate counts and the dirty-range count
+  // and the list of vertices this pass's hits change; the list of the
+  // previous pass (the other parity) is read by this pass
+  int* vcount_cur = S.counters + CNT_VLIST + (pass_id & 1);
+  const int* vcount_prev = S.counters + CNT_VLIST + ((pass_id & 1) ^ 1);
   launch_zero_counters(h->st, S,
                        (1u << CNT_PAIRS) | (1u << CNT_HITS) | (1u << CNT_HIT_PAIRS) |
                            (1u << CNT_HIT_SUB) | (1u << CNT_QPAIRS) | (1u << CNT_DIRTY) |
-                           (0x3fu << CNT_CAND64) | (1u << CNT_DRANGES));
+                           (0x3fu << CNT_CAND64) | (1u << CNT_DRANGES) |
+                           (1u << (CNT_VLIST + (pass_id & 1))));
   NarrowParams NP;
   NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
   NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
@@ -280,14 +302,18 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   int nq = 0;
   int nhits = 0, nhp = 0;
   int narrow_n = 0;
-  if (!incremental) {
-    HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
-    launch_element_boxes(h->st, S, ccd, P.dt);
+  if (!incremental && !seeded) {
+    if (dual) {  // static boxes for this pass, swept boxes for the first CCD pass
+      launch_element_boxes(h->st, S, 0, P.dt, S.pbox);
+      launch_element_boxes(h->st, S, 1, P.dt, S.ebox);
+    } else {
+      launch_element_boxes(h->st, S, ccd, P.dt, S.ebox);
+    }
     if (kind == CS_PASS_PROXIMITY) HIPCHK(h, build_bvh(h->st, S));
     HIPCHK(h, refit_bvh(h->st, S));
     for (;;) {  // traversal; re-run with a larger stream if it overflowed
       mark(h, &T.t0);
-      launch_traverse(h->st, S, S.pair_cap);
+      launch_traverse(h->st, S, dual ? S.pairs_seed : S.pairs, S.pair_cap, dual ? S.pbox : nullptr);
       mark(h, &T.t1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
@@ -295,13 +321,88 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       npairs = h->h_counters[CNT_PAIRS];
       if (npairs <= S.pair_cap) break;
       S.pair_cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
-      dfree(S.pairs); dfree(S.pairs_prev);
+      dfree(S.pairs); dfree(S.pairs_prev); dfree(S.pairs_seed);
       HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
       HIPCHK(h, dalloc(&S.pairs_prev, S.pair_cap));
+      HIPCHK(h, dalloc(&S.pairs_seed, S.pair_cap));
       HIPCHK(h, hipMemsetAsync(S.counters + CNT_PAIRS, 0, sizeof(int), h->st));
       HIPCHK(h, hipMemsetAsync(S.counters + CNT_CAND64, 0, 2 * sizeof(int), h->st));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_PCAND64, 0, 2 * sizeof(int), h->st));
     }
     cand = c64(h, CNT_CAND64);
+    const int2* stream = S.pairs;
+    const int* stream_dev = nullptr;
+    if (dual) {
+      // this pass's pairs are the flagged ones; their number is read on the
+      // device by the narrow phase (at most npairs)
+      h->seed.valid = true;
+      h->seed.npairs = npairs;
+      h->seed.cand = cand;
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_PPAIRS, 0, sizeof(int), h->st));
+      launch_split_seed(h->st, S, npairs);
+      stream_dev = S.counters + CNT_PPAIRS;
+    }
+    mark(h, &T.e1);
+    for (;;) {
+      mark(h, &T.n0);
+      launch_narrow(h->st, S, NP, stream, npairs, stream_dev, S.pair_cap);
+      mark(h, &T.n1);
+      HIPCHK(h, hipGetLastError());
+      int rc = read_counters(h);
+      if (rc) return rc;
+      nhits = h->h_counters[CNT_HITS];
+      nhp = h->h_counters[CNT_HIT_PAIRS];
+      if (nhits <= S.hit_cap && nhp <= S.hp_cap) break;
+      HIPCHK(h, grow_hit_buffers(h, nhits, nhp, 0, 0));
+      if ((rc = set_counter(h, CNT_HITS, 0))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_PAIRS, 0))) return rc;
+      if ((rc = set_counter(h, CNT_HIT_SUB, 0))) return rc;
+    }
+    if (dual) {  // (counted by the split, read with the narrow phase's counters)
+      npairs = h->h_counters[CNT_PPAIRS];
+      cand = c64(h, CNT_PCAND64);
+    }
+    narrow_n = npairs;
+  } else if (seeded) {
+    // The swept-box pairs of the proximity pass, with the elements whose
+    // vertices that pass's hits changed queried again under their new boxes
+    // (the incremental scheme below); no narrow-phase result carries over
+    // (the proximity pass ran another test), so every pair is tested.
+    launch_mark_dirty(h->st, S, vcount_prev, 0);
+    mark(h, &T.t0);
+    if (exact_counts) launch_query_dirty(h->st, S, 0);  // seed boxes: count only
+    launch_dirty_element_boxes(h->st, S, P.dt);
+    launch_grow_dirty(h->st, S);
+    launch_query_dirty(h->st, S, 1);
+    mark(h, &T.t1);
+    launch_carry_pairs_from(h->st, S, S.pairs_seed, seed.npairs);
+    for (int attempt = 0;; ++attempt) {
+      int rc = read_counters(h);
+      if (rc) return rc;
+      nq = h->h_counters[CNT_QPAIRS];
+      if (nq <= S.qpair_cap) break;
+      if (attempt > 4) return fail(h, CS_ERR_INTERNAL, "query stream keeps overflowing");
+      S.qpair_cap = nq + nq / 2 + 1024;
+      dfree(S.qpairs);
+      HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_QPAIRS, 0, sizeof(int), h->st));
+      HIPCHK(h, hipMemsetAsync(S.counters + CNT_NEW64, 0, 2 * sizeof(int), h->st));
+      launch_query_dirty(h->st, S, 1);
+    }
+    carried_pairs = h->h_counters[CNT_PAIRS];
+    npairs = carried_pairs + nq;
+    if (npairs > S.pair_cap) {
+      const int cap = (int)std::min<int64_t>((int64_t)npairs + npairs / 4 + 1024, INT32_MAX / 2);
+      HIPCHK(h, grow_keep(&S.pairs, carried_pairs, cap, h->st));
+      dfree(S.pairs_prev);
+      HIPCHK(h, dalloc(&S.pairs_prev, cap));
+      HIPCHK(h, grow_keep(&S.pairs_seed, 0, cap, h->st));
+      S.pair_cap = cap;
+    }
+    if (nq > 0)
+      HIPCHK(h, hipMemcpyAsync(S.pairs + carried_pairs, S.qpairs, sizeof(int2) * (size_t)nq,
+                               hipMemcpyDeviceToDevice, h->st));
+    cand = exact_counts ? seed.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64) : ~0ull;
     mark(h, &T.e1);
     narrow_n = npairs;
     for (;;) {
@@ -326,8 +427,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // phase appends to them, and overflows are detected afterwards (the
     // narrow phase only appends records and performs idempotent zone
     // merges, so it can be run again).
-    launch_mark_dirty(h->st, S);
-    HIPCHK(h, hipMemsetAsync(S.dirty_v, 0, S.nv, h->st));
+    launch_mark_dirty(h->st, S, vcount_prev, prev.nzone);
     mark(h, &T.t0);
     // previous boxes: count only.  The count only feeds the exact candidate
     // total of the pass (a diagnostic), so it runs when the caller asks for
@@ -392,6 +492,8 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       HIPCHK(h, grow_keep(&S.pairs, carried_pairs, cap, h->st));
       dfree(S.pairs_prev);
       HIPCHK(h, dalloc(&S.pairs_prev, cap));
+      dfree(S.pairs_seed);  // (its pairs were used by the first CCD pass)
+      HIPCHK(h, dalloc(&S.pairs_seed, cap));
       S.pair_cap = cap;
     }
     if (nq > 0 && npairs >= 0)
@@ -401,10 +503,12 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
                                               : ~0ull;
   }
   mark(h, &T.e2);
-  HIPCHK(h, apply_hits(h->st, S, nhits));
+  HIPCHK(h, apply_hits(h->st, S, nhits, vcount_cur));
+  int nzone = 0;
   if (kind == CS_PASS_ZONE) {
+    nzone = h->h_counters[CNT_ZLIST];
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), h->st));
-    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, h->h_counters[CNT_ZLIST]));
+    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, nzone));
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark(h, &T.e3);
@@ -414,7 +518,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   pi.candidates = cand == ~0ull ? -1 : (int64_t)cand;
   pi.narrow_pairs = npairs;
   pi.tested_pairs = narrow_n;
-  pi.incremental = incremental ? 1 : 0;
+  pi.incremental = (incremental || seeded) ? 1 : 0;
   pi.hit_pairs = nhp;
   pi.hit_subtests = h->h_counters[CNT_HIT_SUB];
   if (P.log_pairs) {  // the pair log is read after every pass
@@ -430,8 +534,9 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     h->prev.npairs = npairs;
     h->prev.nhits = nhits;
     h->prev.nhp = nhp;
-    h->prev.nq = incremental ? nq : 0;
+    h->prev.nq = (incremental || seeded) ? nq : 0;
     h->prev.cand = cand;
+    h->prev.nzone = nzone;
   }
   return device_error(h);
 }
@@ -662,6 +767,8 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   S.pair_cap = (int)std::min<int64_t>(3ll * ne + 4096, INT32_MAX / 2);
   HIPCHK(h, dalloc(&S.pairs, S.pair_cap));
   HIPCHK(h, dalloc(&S.pairs_prev, S.pair_cap));
+  HIPCHK(h, dalloc(&S.pairs_seed, S.pair_cap));
+  HIPCHK(h, dalloc(&S.pbox, ne));
   S.qpair_cap = std::max(1 << 16, ne / 8);
   HIPCHK(h, dalloc(&S.qpairs, S.qpair_cap));
   HIPCHK(h, alloc_hits(S, std::max(1 << 16, ne / 4), 0, h->st));
@@ -669,9 +776,28 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.hp, S.hp_cap));
   HIPCHK(h, dalloc(&S.hp_prev, S.hp_cap));
   HIPCHK(h, dalloc(&S.leaf_of, ne));
-  HIPCHK(h, dalloc(&S.dirty_v, nv));
-  HIPCHK(h, dalloc(&S.dirty_e, ne));
+  HIPCHK(h, dalloc(&S.dirty_e, ((size_t)ne + 3) & ~(size_t)3));
   HIPCHK(h, dalloc(&S.dlist, ne));
+  HIPCHK(h, dalloc(&S.vlist, nv));
+  // vertex -> elements, in element order
+  std::vector<int32_t> vptr((size_t)nv + 1, 0), vadj;
+  for (int e = 0; e < ne; ++e) {
+    vptr[(size_t)ev[e].x + 1]++;
+    vptr[(size_t)ev[e].y + 1]++;
+    if (ev[e].z >= 0) vptr[(size_t)ev[e].z + 1]++;
+  }
+  for (int v = 0; v < nv; ++v) vptr[(size_t)v + 1] += vptr[v];
+  vadj.resize(std::max<size_t>(1, (size_t)vptr[nv]));
+  {
+    std::vector<int32_t> fillp(vptr.begin(), vptr.end() - 1);
+    for (int e = 0; e < ne; ++e) {
+      vadj[fillp[ev[e].x]++] = e;
+      vadj[fillp[ev[e].y]++] = e;
+      if (ev[e].z >= 0) vadj[fillp[ev[e].z]++] = e;
+    }
+  }
+  HIPCHK(h, dalloc(&S.vadj_ptr, (size_t)nv + 1));
+  HIPCHK(h, dalloc(&S.vadj, vadj.size()));
   HIPCHK(h, dalloc(&S.dpre, 2 * ne - 1));
   HIPCHK(h, dalloc(&S.tnext, kNumXcd));
   {
@@ -707,7 +833,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.slast, slast.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.counters, 0, sizeof(int) * COUNTER_INTS));
   HIPCHK(h, hipMemset(S.has, 0, nv));
-  HIPCHK(h, hipMemset(S.dirty_v, 0, nv));
+  HIPCHK(h, hipMemset(S.dirty_e, 0, ((size_t)ne + 3) & ~(size_t)3));
+  HIPCHK(h, hipMemcpy(S.vadj_ptr, vptr.data(), sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(S.vadj, vadj.data(), sizeof(int32_t) * vadj.size(), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.dpre, 0, 2 * (size_t)ne - 1));
   HIPCHK(h, hipMemset(S.rflag, 0, sizeof(int32_t) * ((2 * (size_t)ne - 1 + QW_RANGE - 1) / QW_RANGE)));
   h->assembled = true;

@@ -343,18 +343,20 @@ __device__ inline bool f64_overlap(const dbox& a, const dbox& b) {  // AABB::isC
          (a.lo[2] <= b.hi[2] && a.hi[2] >= b.lo[2]);
 }
 
-// Reference early rejects for the canonical orientation (a = hi, b = lo).
-__device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
+// Reference early rejects for the canonical orientation (a = hi, b = lo):
+// 0 kept, REJ_RIGID same rigid surface, REJ_SHARED a shared vertex.
+constexpr int REJ_RIGID = 1, REJ_SHARED = 2;
+__device__ inline int pair_reject_kind(int4 ta, int4 tb, uint8_t fa) {
   const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
   if (tri_a && tri_b) {
-    if (ta.w == tb.w && (fa & EF_RIGID)) return true;  // same rigid surface
+    if (ta.w == tb.w && (fa & EF_RIGID)) return REJ_RIGID;  // same rigid surface
     const int va[3] = {ta.x, ta.y, ta.z}, vb[3] = {tb.x, tb.y, tb.z};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        if (va[i] == vb[k]) return true;
-    return false;
+        if (va[i] == vb[k]) return REJ_SHARED;
+    return 0;
   }
   if (!tri_a && !tri_b) {
     const int p[4] = {ta.x, ta.y, tb.x, tb.y};
@@ -362,15 +364,18 @@ __device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int k = i + 1; k < 4; ++k)
-        if (p[i] == p[k]) return true;
-    return false;
+        if (p[i] == p[k]) return REJ_SHARED;
+    return 0;
   }
   const int4 T = tri_a ? ta : tb, B = tri_a ? tb : ta;
   const int vt[3] = {T.x, T.y, T.z};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
-    if (vt[i] == B.x || vt[i] == B.y) return true;
-  return false;
+    if (vt[i] == B.x || vt[i] == B.y) return REJ_SHARED;
+  return 0;
+}
+__device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
+  return pair_reject_kind(ta, tb, fa) != 0;
 }
 
 constexpr int TRAV_THREADS = 256;
@@ -383,18 +388,27 @@ constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flu
 // counter) while the others go on; each XCD starts on its own contiguous
 // eighth of the leaves (L2 locality) and then helps the others.  Pairs are
 // buffered per lane and written by the whole wavefront when a buffer fills.
+// DUAL: the boxes are the swept boxes of the first CCD pass; the walk also
+// counts the static-box overlaps (the proximity pass's boxes, each inside
+// its element's swept box) among the pairs it rejects: a pair with a shared
+// vertex always overlaps (both static boxes hold that vertex), a pair on one
+// rigid surface is tested.  The kept pairs are tested by k_split_seed, so
+// one walk serves both passes.
 constexpr int PT_BLOCKS_PER_CU = 8;
 constexpr int PT_REFILL = 16;   // refill once this many lanes are idle
 constexpr int PT_STEPS = 16;    // walk steps between refill checks
 
+template <bool DUAL>
 __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
                                                            const int32_t* __restrict__ lpre,
                                                            const dbox* __restrict__ ebox,
+                                                           const dbox* __restrict__ pbox,
                                                            const int4* __restrict__ ev,
                                                            const uint8_t* __restrict__ eflags,
                                                            int n, int2* __restrict__ pairs, int cap,
                                                            int* __restrict__ pair_count,
                                                            unsigned long long* __restrict__ cand_count,
+                                                           unsigned long long* __restrict__ pcand_count,
                                                            int* __restrict__ next) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
   const int t = threadIdx.x, lane = t & 63;
@@ -403,7 +417,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
   // to the XCDs round-robin)
   int chunk = (int)(blockIdx.x % kNumXcd), tried = 0;
   auto chunk_end = [&](int c) { return (int)(((int64_t)n * (c + 1)) / kNumXcd); };
-  unsigned long long cand = 0;
+  unsigned long long cand = 0, pcand = 0;
   int nbuf = 0;
   int nd = -1;  // current node of this lane's walk, -1: idle
   int e = 0;
@@ -488,8 +502,12 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
       const uint8_t of = eflags[e2];
       if (!f64_overlap(ob, mb)) continue;
       ++cand;
-      const bool rej = (e > e2) ? pair_rejected(mt, ot, mf) : pair_rejected(ot, mt, of);
-      if (rej) continue;
+      const int rk = (e > e2) ? pair_reject_kind(mt, ot, mf) : pair_reject_kind(ot, mt, of);
+      if (rk) {
+        if constexpr (DUAL)
+          if (rk == REJ_SHARED || f64_overlap(pbox[e], pbox[e2])) ++pcand;
+        continue;
+      }
       buf[nbuf * TRAV_THREADS + t] = make_int2(min(e, e2), max(e, e2));
       if (++nbuf == TRAV_BUF) break;  // the buffer is written before the walk goes on
     }
@@ -500,6 +518,58 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if (lane == 0 && c) atomicAdd(cand_count, c);
+  if constexpr (DUAL) {
+    unsigned long long pc = pcand;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o);
+    if (lane == 0 && pc) atomicAdd(pcand_count, pc);
+  }
+}
+
+// The swept-box pairs of the dual traversal whose static boxes overlap are
+// the proximity pass's stream (and are counted with its candidates).  One
+// tile of SPLIT_TILE pairs per work-group, one atomic per tile.
+constexpr int SPLIT_ITEMS = 16;
+constexpr int SPLIT_TILE = 256 * SPLIT_ITEMS;
+__global__ void __launch_bounds__(256) k_split_seed(const int2* __restrict__ seed, int n,
+                                                    const dbox* __restrict__ pbox,
+                                                    int2* __restrict__ prox,
+                                                    int* __restrict__ counters) {
+  __shared__ int s_off[256];
+  __shared__ int s_base;
+  const int t = threadIdx.x;
+  const int base_i = blockIdx.x * SPLIT_TILE;
+  int2 p[SPLIT_ITEMS];
+  uint32_t keep = 0u;
+#pragma unroll
+  for (int k = 0; k < SPLIT_ITEMS; ++k) {
+    const int i = base_i + k * 256 + t;
+    p[k] = make_int2(0, 0);
+    if (i < n) {
+      p[k] = seed[i];
+      if (f64_overlap(pbox[p[k].x], pbox[p[k].y])) keep |= 1u << k;
+    }
+  }
+  const int mine = __popc(keep);
+  s_off[t] = mine;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {  // inclusive scan of the counts
+    const int y = t >= off ? s_off[t - off] : 0;
+    __syncthreads();
+    s_off[t] += y;
+    __syncthreads();
+  }
+  if (t == 255) {
+    s_base = s_off[255] ? atomicAdd(&counters[CNT_PPAIRS], s_off[255]) : 0;
+    if (s_off[255])
+      atomicAdd(reinterpret_cast<unsigned long long*>(counters + CNT_PCAND64),
+                (unsigned long long)s_off[255]);
+  }
+  __syncthreads();
+  int o = s_base + s_off[t] - mine;
+#pragma unroll
+  for (int k = 0; k < SPLIT_ITEMS; ++k)
+    if ((keep >> k) & 1u) prox[o++] = p[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -514,9 +584,19 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
 // elements under the previous boxes) + (the same under the new boxes).
 
 
-// dirty_e[e] = element has a dirty vertex; list of dirty elements
-__global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
-                                                    const uint8_t* __restrict__ dirty_v, int ne,
+// dirty_e[e] = element has a dirty vertex; list of dirty elements, each
+// once.  The dirty vertices are those the last pass changed: the vertices of
+// its hits (vlist, nvl of them) and, after a zone pass, the movable members
+// of the impact zones (zlist, nzone of them).  Their elements are found
+// through the vertex -> element table, one thread per (vertex, MD_SLOTS-th
+// element); a word atomic on the element's flag byte keeps the first finder.
+constexpr int MD_SLOTS = 8;
+__global__ void __launch_bounds__(256) k_mark_dirty(const int32_t* __restrict__ vlist,
+                                                    const int* __restrict__ vcount,
+                                                    const uint32_t* __restrict__ zlist, int nzone,
+                                                    const uint8_t* __restrict__ fixed,
+                                                    const int32_t* __restrict__ vadj_ptr,
+                                                    const int32_t* __restrict__ vadj,
                                                     uint8_t* __restrict__ dirty_e,
                                                     int32_t* __restrict__ dlist,
                                                     const int32_t* __restrict__ enode,
@@ -524,22 +604,46 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int4* __restrict__ ev,
                                                     int32_t* __restrict__ rflag,
                                                     int32_t* __restrict__ rlist,
                                                     int* __restrict__ counters) {
-  __shared__ int s_cnt, s_base;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  bool d = false;
-  if (e < ne) {
-    const int4 t = ev[e];
-    d = dirty_v[t.x] | dirty_v[t.y] | ((t.z >= 0) ? dirty_v[t.z] : 0);
-    dirty_e[e] = d ? 1 : 0;
-    if (d) {  // the leaf, and its QW_RANGE-node range of the pre-order layout
-      const int p = enode[e];
-      dpre[p] = 1;
-      const int r = p >> QW_RANGE_SHIFT;
-      if (atomicExch(&rflag[r], 1) == 0) rlist[atomicAdd(&counters[CNT_DRANGES], 1)] = r;
+  const int nvl = *vcount;
+  const long long total = (long long)(nvl + nzone) * MD_SLOTS;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  // whole wavefronts iterate together (the appends are per wavefront)
+  for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < total; t0 += stride) {
+    const long long t = t0 + threadIdx.x;
+    int a = 0, a1 = 0;
+    if (t < total) {
+      const int k = (int)(t / MD_SLOTS);
+      const int v = k < nvl ? vlist[k] : (int)zlist[k - nvl];
+      if (k < nvl || !fixed[v]) {
+        a = vadj_ptr[v] + (int)(t % MD_SLOTS);
+        a1 = vadj_ptr[v + 1];
+      }
+    }
+    for (;; a += MD_SLOTS) {
+      const bool live = a < a1;
+      if (!__ballot(live)) break;
+      int e = 0;
+      bool first = false;
+      if (live) {
+        e = vadj[a];
+        const uint32_t bit = 1u << (8 * (e & 3));
+        first = !(atomicOr(reinterpret_cast<uint32_t*>(dirty_e + (e & ~3)), bit) & bit);
+      }
+      const int slot = wave_append(first, &counters[CNT_DIRTY]);
+      bool new_range = false;
+      int r = 0;
+      if (first) {
+        dlist[slot] = e;
+        // the leaf, and its QW_RANGE-node range of the pre-order layout
+        const int p = enode[e];
+        dpre[p] = 1;
+        r = p >> QW_RANGE_SHIFT;
+        new_range = atomicExch(&rflag[r], 1) == 0;
+      }
+      const int rs = wave_append(new_range, &counters[CNT_DRANGES]);
+      if (new_range) rlist[rs] = r;
     }
   }
-  const int slot = block_append(d, &counters[CNT_DIRTY], &s_cnt, &s_base);
-  if (d) dlist[slot] = e;
 }
 
 __global__ void k_dirty_boxes(const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
@@ -864,6 +968,8 @@ __global__ void __launch_bounds__(64) k_query_ranges(
 __global__ void __launch_bounds__(64) k_clear_ranges(const int32_t* __restrict__ rlist, int npre,
                                                      uint8_t* __restrict__ dpre,
                                                      int32_t* __restrict__ rflag,
+                                                     const int32_t* __restrict__ dlist,
+                                                     uint8_t* __restrict__ dirty_e,
                                                      const int* __restrict__ counters) {
   const int nr = counters[CNT_DRANGES];
   for (int ri = blockIdx.x; ri < nr; ri += gridDim.x) {
@@ -872,6 +978,10 @@ __global__ void __launch_bounds__(64) k_clear_ranges(const int32_t* __restrict__
     if ((int)threadIdx.x < QW_RANGE && p < npre) dpre[p] = 0;
     if (threadIdx.x == 0) rflag[r] = 0;
   }
+  // the dirty elements' flags
+  const int nd = counters[CNT_DIRTY];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x)
+    dirty_e[dlist[k]] = 0;
 }
 
 // carry clean pairs / hits / hit-pair records of the previous pass
@@ -951,15 +1061,17 @@ __global__ void __launch_bounds__(256) k_carry_hp(const int4* __restrict__ prev,
   }
 }
 
-void launch_mark_dirty(hipStream_t st, DevState& S) {
-  hipLaunchKernelGGL(k_mark_dirty, dim3((S.ne + 255) / 256), dim3(256), 0, st, S.ev, S.dirty_v,
-                     S.ne, S.dirty_e, S.dlist, S.leaf_of, S.dpre, S.rflag, S.rlist, S.counters);
+void launch_mark_dirty(hipStream_t st, DevState& S, const int* vcount, int nzone) {
+  // grid for a typical list; longer lists are walked grid-stride
+  hipLaunchKernelGGL(k_mark_dirty, dim3(std::min((MD_SLOTS * S.nv + 255) / 256, 256)), dim3(256),
+                     0, st, S.vlist, vcount, S.zlist, nzone, S.fixed, S.vadj_ptr, S.vadj,
+                     S.dirty_e, S.dlist, S.leaf_of, S.dpre, S.rflag, S.rlist, S.counters);
 }
 
 void launch_clear_ranges(hipStream_t st, DevState& S) {
   const int npre = 2 * S.ne - 1;
   hipLaunchKernelGGL(k_clear_ranges, dim3(std::min((npre + 63) / 64, 2048)), dim3(64), 0, st,
-                     S.rlist, npre, S.dpre, S.rflag, S.counters);
+                     S.rlist, npre, S.dpre, S.rflag, S.dlist, S.dirty_e, S.counters);
 }
 
 // zero the counters whose bits are set in mask (one launch for several)
@@ -992,6 +1104,12 @@ void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
                      emit, S.qpairs, S.qpair_cap, S.counters);
 }
 
+void launch_carry_pairs_from(hipStream_t st, DevState& S, const int2* src, int n) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_carry_pairs, dim3((n + 256 * CARRY_ITEMS - 1) / (256 * CARRY_ITEMS)), dim3(256), 0, st,
+                       src, n, S.dirty_e, S.pairs, S.pair_cap, S.counters);
+}
+
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev) {
   if (npairs_prev > 0)
     hipLaunchKernelGGL(k_carry_pairs, dim3((npairs_prev + 256 * CARRY_ITEMS - 1) / (256 * CARRY_ITEMS)), dim3(256), 0, st,
@@ -1005,11 +1123,11 @@ void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, 
 }
 
 // ---------------------------------------------------------------------------
-void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt) {
+void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt, dbox* out) {
   const int ne = S.ne;
   if (ne == 0) return;
   hipLaunchKernelGGL(k_element_boxes, dim3((ne + 255) / 256), dim3(256), 0, st, S.ev, S.x, S.x_old,
-                     S.av, dt, moving, ne, S.ebox);
+                     S.av, dt, moving, ne, out);
 }
 
 hipError_t build_bvh(hipStream_t st, DevState& S) {
@@ -1075,14 +1193,27 @@ hipError_t refit_bvh(hipStream_t st, DevState& S) {
   return hipGetLastError();
 }
 
-void launch_traverse(hipStream_t st, DevState& S, int cap) {
+void launch_traverse(hipStream_t st, DevState& S, int2* out, int cap, const dbox* pbox) {
   const int n = S.ne;
   if (n < 2) return;
   (void)hipMemsetAsync(S.tnext, 0, sizeof(int) * kNumXcd, st);
   const int blocks = std::min(S.num_cus * PT_BLOCKS_PER_CU, (n + TRAV_THREADS - 1) / TRAV_THREADS);
-  hipLaunchKernelGGL(k_traverse, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st, S.nodes, S.lpre,
-                     S.ebox, S.ev, S.eflags, n, S.pairs, cap, S.counters + CNT_PAIRS,
-                     (unsigned long long*)(S.counters + CNT_CAND64), S.tnext);
+  if (pbox)
+    hipLaunchKernelGGL(k_traverse<true>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
+                       S.nodes, S.lpre, S.ebox, pbox, S.ev, S.eflags, n, out, cap,
+                       S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
+                       (unsigned long long*)(S.counters + CNT_PCAND64), S.tnext);
+  else
+    hipLaunchKernelGGL(k_traverse<false>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
+                       S.nodes, S.lpre, S.ebox, nullptr, S.ev, S.eflags, n, out, cap,
+                       S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
+                       nullptr, S.tnext);
+}
+
+void launch_split_seed(hipStream_t st, DevState& S, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_split_seed, dim3((n + SPLIT_TILE - 1) / SPLIT_TILE), dim3(256), 0, st,
+                     S.pairs_seed, n, S.pbox, S.pairs, S.counters);
 }
 
 }  // namespace cs

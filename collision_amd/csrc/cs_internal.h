@@ -47,8 +47,12 @@ enum {
   CNT_NEW64 = 20,        // candidates touching dirty elements, current boxes
   CNT_ZLIST = 22,        // vertices in the zone member list (kept for the whole step)
   CNT_DRANGES = 23,      // tree ranges holding dirty leaves (kept until the next pass clears them)
+  CNT_VLIST = 24,        // 24, 25: vertices whose average velocity a pass changed through
+                        // hits (dirty vertex list), by pass parity
   CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
-  CNT_INTS = 32
+  CNT_PCAND64 = 32,      // 32, 33: static-box overlaps found by the swept-box traversal
+  CNT_PPAIRS = 34,       // pairs of that traversal whose static boxes overlap
+  CNT_INTS = 40
 };
 
 struct NarrowParams {
@@ -134,9 +138,13 @@ struct DevState {
   int4* hp_prev = nullptr;
   int hp_cap = 0;
   // vertices / elements whose average velocity changed since the last narrow phase
-  uint8_t* dirty_v = nullptr;
-  uint8_t* dirty_e = nullptr;
+  uint8_t* dirty_e = nullptr;   // padded to whole 32-bit words (word atomics mark it)
   int32_t* dlist = nullptr;
+  dbox* pbox = nullptr;         // static (proximity) boxes, beside the swept ones in ebox
+  int2* pairs_seed = nullptr;   // swept-box candidate pairs of the proximity pass
+  int32_t* vlist = nullptr;     // vertices the last apply changed, each once
+  int32_t* vadj_ptr = nullptr;  // vertex -> elements holding it (CSR over all elements)
+  int32_t* vadj = nullptr;
   uint64_t *hkey = nullptr, *hkey_tmp = nullptr;
   uint32_t *hidx = nullptr, *hidx_tmp = nullptr;
   uint32_t *rkey = nullptr, *rkey_tmp = nullptr, *rval = nullptr, *rval_tmp = nullptr;
@@ -178,13 +186,21 @@ hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64
                           bool* result_in_tmp);
 
 // broad phase
-void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt);
+void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt, dbox* out);
 void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt);
 hipError_t build_bvh(hipStream_t st, DevState& S);  // also finds the refit top nodes
 hipError_t refit_bvh(hipStream_t st, DevState& S);
-void launch_traverse(hipStream_t st, DevState& S, int cap);
+// candidate pairs of the current boxes (ebox) into out; with pbox (static
+// boxes, each inside its element's swept box in ebox), also the static-box
+// overlaps among the rejected pairs (CNT_PCAND64)
+void launch_traverse(hipStream_t st, DevState& S, int2* out, int cap, const dbox* pbox);
+// the pairs of S.pairs_seed[0..n) whose static boxes overlap into S.pairs
+// (count CNT_PPAIRS, also added to CNT_PCAND64)
+void launch_split_seed(hipStream_t st, DevState& S, int n);
 // incremental passes
-void launch_mark_dirty(hipStream_t st, DevState& S);
+// elements of the vertices the previous pass changed: *vcount hit vertices
+// in vlist and nzone zone members in zlist
+void launch_mark_dirty(hipStream_t st, DevState& S, const int* vcount, int nzone);
 // clears the previous pass's dirty-range marks; its range count is zeroed by
 // the caller afterwards (launch_zero_counters)
 void launch_clear_ranges(hipStream_t st, DevState& S);
@@ -194,6 +210,8 @@ constexpr int QW_RANGE = 1 << QW_RANGE_SHIFT;  // pre-order nodes per dirty-quer
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
 void launch_grow_dirty(hipStream_t st, DevState& S);
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);
+// pairs of src[0..n) with no dirty element into S.pairs (count CNT_PAIRS)
+void launch_carry_pairs_from(hipStream_t st, DevState& S, const int2* src, int n);
 
 // narrow phase
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
@@ -202,7 +220,7 @@ void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd);
 void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id);
 
 // response
-hipError_t apply_hits(hipStream_t st, DevState& S, int nhits);
+hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount);
 void launch_average_velocity(hipStream_t st, DevState& S, double dt);
 void launch_predict(hipStream_t st, DevState& S, double dt);
 void launch_uf_init(hipStream_t st, DevState& S);
@@ -230,6 +248,19 @@ __device__ inline int xcd_block(int b, int G) {
   const int q = G / kNumXcd, r = G % kNumXcd;
   const int x = b % kNumXcd, i = b / kNumXcd;
   return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// slot in a list for the lanes with want set, one atomic per wavefront;
+// every active lane of the wavefront calls it
+__device__ inline int wave_append(bool want, int* counter) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return -1;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader);
+  return want ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
 __device__ inline int block_append(bool want, int* counter, int* s_cnt, int* s_base) {

@@ -88,36 +88,45 @@ __global__ void k_expand_records(const hit_rec* __restrict__ hits,
   rval[i] = h * 4u + (uint32_t)p;
 }
 
+// One thread per vertex segment of the sorted records.  The vertex joins the
+// list of vertices whose average velocity this pass changed (vcount counts
+// it; each vertex heads one segment).
 __global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __restrict__ rval,
                         int nrec, int nv, const hit_rec* __restrict__ hits,
                         dvec4* __restrict__ av, uint8_t* __restrict__ has,
-                        uint8_t* __restrict__ dirty_v, int* __restrict__ err) {
+                        int32_t* __restrict__ vlist, int* __restrict__ vcount,
+                        int* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrec) return;
-  const uint32_t v = rkey[i];
-  if (v >= (uint32_t)nv) return;
-  if (i > 0 && rkey[i - 1] == v) return;  // not the head of the vertex segment
-  double imp[3] = {0.0, 0.0, 0.0}, fric[3] = {0.0, 0.0, 0.0};
-  int count = 0;
-  for (int j = i; j < nrec && rkey[j] == v; ++j) {
-    const uint32_t hv = rval[j];
-    const hit_rec& h = hits[hv >> 2];
-    const int p = hv & 3;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      imp[k] += h.imp[p][k];
-      fric[k] += h.fric[p][k];
-    }
-    ++count;
+  uint32_t v = 0;
+  bool head = false;
+  if (i < nrec) {
+    v = rkey[i];
+    head = v < (uint32_t)nv && !(i > 0 && rkey[i - 1] == v);  // head of the vertex segment
   }
-  dvec4 a = av[v];
-  a.x += (imp[0] + fric[0]) / count;
-  a.y += (imp[1] + fric[1]) / count;
-  a.z += (imp[2] + fric[2]) / count;
-  if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(err, CS_DEVERR_NONFINITE);
-  av[v] = a;
-  has[v] = 1;
-  dirty_v[v] = 1;
+  if (head) {
+    double imp[3] = {0.0, 0.0, 0.0}, fric[3] = {0.0, 0.0, 0.0};
+    int count = 0;
+    for (int j = i; j < nrec && rkey[j] == v; ++j) {
+      const uint32_t hv = rval[j];
+      const hit_rec& h = hits[hv >> 2];
+      const int p = hv & 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        imp[k] += h.imp[p][k];
+        fric[k] += h.fric[p][k];
+      }
+      ++count;
+    }
+    dvec4 a = av[v];
+    a.x += (imp[0] + fric[0]) / count;
+    a.y += (imp[1] + fric[1]) / count;
+    a.z += (imp[2] + fric[2]) / count;
+    if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(err, CS_DEVERR_NONFINITE);
+    av[v] = a;
+    has[v] = 1;
+  }
+  const int slot = wave_append(head, vcount);
+  if (head) vlist[slot] = (int32_t)v;
 }
 
 static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rounded to 8
@@ -127,7 +136,7 @@ static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rou
   return b < 8 ? 8 : b;
 }
 
-hipError_t apply_hits(hipStream_t st, DevState& S, int nhits) {
+hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount) {
   if (nhits <= 0) return hipSuccess;
   const int nb = (nhits + 255) / 256;
   hipLaunchKernelGGL(k_hit_keys, dim3(nb), dim3(256), 0, st, S.hits, nhits, S.hkey, S.hidx);
@@ -144,7 +153,7 @@ hipError_t apply_hits(hipStream_t st, DevState& S, int nhits) {
   const uint32_t* rk = in_tmp ? S.rkey_tmp : S.rkey;
   const uint32_t* rv = in_tmp ? S.rval_tmp : S.rval;
   hipLaunchKernelGGL(k_apply, dim3((nrec + 255) / 256), dim3(256), 0, st, rk, rv, nrec, S.nv,
-                     S.hits, S.av, S.has, S.dirty_v, S.counters + CNT_ERR);
+                     S.hits, S.av, S.has, S.vlist, vcount, S.counters + CNT_ERR);
   return hipGetLastError();
 }
 
@@ -208,7 +217,7 @@ __global__ void __launch_bounds__(ZONE_LANES)
                   const int32_t* __restrict__ zhead, const int32_t* __restrict__ zend,
                   const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
                   const uint8_t* __restrict__ fixed, double m, double dt,
-                  int* __restrict__ counters, uint8_t* __restrict__ dirty_v) {
+                  int* __restrict__ counters) {
   __shared__ double part[ZONE_LANES][9];
   int* err = counters + CNT_ERR;
   const int l = threadIdx.x;
@@ -345,7 +354,6 @@ __global__ void __launch_bounds__(ZONE_LANES)
       }
       if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
       av[p] = dvec4{an[0], an[1], an[2], 0.0};
-      dirty_v[p] = 1;
     }
   }
 }
@@ -369,7 +377,7 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nl
   hipLaunchKernelGGL(k_zone_heads, dim3(nb), dim3(256), 0, st, zk, nlist, vbits, S.zkey, S.zhead,
                      S.zend, S.counters);
   hipLaunchKernelGGL(k_zone_update, dim3(std::min(nlist, 2048)), dim3(ZONE_LANES), 0, st, S.zkey,
-                     zv, S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters, S.dirty_v);
+                     zv, S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters);
   return hipGetLastError();
 }
 

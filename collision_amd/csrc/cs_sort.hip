@@ -312,12 +312,55 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_sort_small(K* __restrict__ ke
   }
 }
 
+// Tiny arrays (n <= RANK_SORT_MAX): one key per thread, whose place in the
+// stable order is the number of keys before it (smaller keys, and equal keys
+// at smaller indices).  Every thread reads all keys from LDS as broadcasts;
+// keys below the thread's own wavefront's 64 have smaller indices (count
+// keys <= it), keys above have larger ones (count keys < it), and only the
+// wavefront's own 64 compare indices.  The result is the stable LSD order of
+// the radix paths.
+constexpr int RANK_SORT_MAX = 1024;
+
+template <typename K>
+__global__ void __launch_bounds__(RANK_SORT_MAX) k_sort_rank(K* __restrict__ keys,
+                                                             uint32_t* __restrict__ vals, int n) {
+  __shared__ K kb[RANK_SORT_MAX];
+  const int i = threadIdx.x;
+  const int w0 = i & ~63, w1 = min(n, w0 + 64);
+  K ki = 0;
+  uint32_t vi = 0;
+  if (i < n) {
+    ki = keys[i];
+    vi = vals[i];
+    kb[i] = ki;
+  }
+  __syncthreads();
+  int rank = 0;
+  if (i < n) {
+    for (int j = 0; j < w0; ++j) rank += kb[j] <= ki ? 1 : 0;  // j < i
+    for (int j = w0; j < w1; ++j) {
+      const K kj = kb[j];
+      rank += (kj < ki || (kj == ki && j < i)) ? 1 : 0;
+    }
+    for (int j = w1; j < n; ++j) rank += kb[j] < ki ? 1 : 0;  // j > i
+  }
+  __syncthreads();  // every thread has read its key and value
+  if (i < n) {
+    keys[rank] = ki;
+    vals[rank] = vi;
+  }
+}
+
 template <typename K>
 static hipError_t radix_sort_impl(hipStream_t st, K* keys, uint32_t* vals, K* tmp_keys,
                                   uint32_t* tmp_vals, uint32_t* hist, uint32_t* partials, int n,
                                   int bits, bool* result_in_tmp) {
   *result_in_tmp = false;
   if (n <= 1 || bits <= 0) return hipSuccess;
+  if (n <= RANK_SORT_MAX) {
+    hipLaunchKernelGGL(k_sort_rank<K>, dim3(1), dim3((n + 63) & ~63), 0, st, keys, vals, n);
+    return hipGetLastError();
+  }
   if (n <= SMALL_SORT_MAX) {
     hipLaunchKernelGGL(k_sort_small<K>, dim3(1), dim3(SMALL_THREADS), 0, st, keys, vals, n, bits);
     return hipGetLastError();
