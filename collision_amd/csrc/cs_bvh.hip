@@ -333,14 +333,18 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// (no short circuit, as f64_overlap below)
 __device__ inline bool f32_overlap(const bvh_node& a, const float lo[3], const float hi[3]) {
-  return (a.lo[0] <= hi[0] && a.hi[0] >= lo[0]) && (a.lo[1] <= hi[1] && a.hi[1] >= lo[1]) &&
-         (a.lo[2] <= hi[2] && a.hi[2] >= lo[2]);
+  return (a.lo[0] <= hi[0]) & (a.hi[0] >= lo[0]) & (a.lo[1] <= hi[1]) & (a.hi[1] >= lo[1]) &
+         (a.lo[2] <= hi[2]) & (a.hi[2] >= lo[2]);
 }
 
-__device__ inline bool f64_overlap(const dbox& a, const dbox& b) {  // AABB::isCollid
-  return (a.lo[0] <= b.hi[0] && a.hi[0] >= b.lo[0]) && (a.lo[1] <= b.hi[1] && a.hi[1] >= b.lo[1]) &&
-         (a.lo[2] <= b.hi[2] && a.hi[2] >= b.lo[2]);
+// AABB::isCollid.  All six comparisons are evaluated (no short circuit), so
+// the compiler keeps the box loads together in one round trip instead of
+// sinking each component's load behind the previous comparison.
+__device__ inline bool f64_overlap(const dbox& a, const dbox& b) {
+  return (a.lo[0] <= b.hi[0]) & (a.hi[0] >= b.lo[0]) & (a.lo[1] <= b.hi[1]) &
+         (a.hi[1] >= b.lo[1]) & (a.lo[2] <= b.hi[2]) & (a.hi[2] >= b.lo[2]);
 }
 
 // Reference early rejects for the canonical orientation (a = hi, b = lo):
@@ -500,6 +504,9 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
       const dbox ob = ebox[e2];
       const int4 ot = ev[e2];
       const uint8_t of = eflags[e2];
+      // (the topology and flags are consumed here so that their loads are
+      // issued with the box's, not after the overlap test)
+      asm volatile("" ::"v"(ot.x), "v"(ot.y), "v"(ot.z), "v"(ot.w), "v"((int)of));
       if (!f64_overlap(ob, mb)) continue;
       ++cand;
       const int rk = (e > e2) ? pair_reject_kind(mt, ot, mf) : pair_reject_kind(ot, mt, of);
