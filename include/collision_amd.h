@@ -83,13 +83,17 @@ typedef struct cs_pass_info {
                                 pass unless log_pairs is set (the exact count needs
                                 an extra query of the changed elements) */
     int64_t narrow_pairs;    /* candidate pairs after the early rejects; -1 in an
-                                incremental pass unless log_pairs is set */
+                                incremental pass unless log_pairs is set (exact in
+                                the first CCD pass, which tests all its pairs) */
     int64_t hit_pairs;       /* pairs whose narrow phase reported contact */
     int64_t hit_subtests;
     int64_t tested_pairs;    /* pairs the narrow-phase kernel evaluated in this pass
-                                (all narrow_pairs in a full pass; only pairs touching
-                                changed vertices in an incremental pass) */
-    int32_t incremental;     /* 1 if the pass reused the previous pass's results */
+                                (all narrow_pairs in a full pass and in the first CCD
+                                pass; only pairs touching changed vertices in later
+                                incremental passes) */
+    int32_t incremental;     /* 1 if the pass reused an earlier pass's results (the
+                                first CCD pass reuses the proximity pass's tree
+                                walk; later passes also its narrow-phase results) */
     int32_t pad;
     double ms_broad, ms_narrow, ms_response;  /* phase wall time on the handle's stream */
     double ms_traverse_kernel;                /* broad-phase kernels (traversal / re-query) */
