@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("dom_hi", C.c_double * 3), ("max_ccd_passes", C.c_int32),
                 ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
                 ("log_pairs", C.c_int32), ("device", C.c_int32),
-                ("force_full_passes", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("force_full_passes", C.c_int32), ("pass_timing", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class MeshView(C.Structure):

@@ -192,6 +192,13 @@ void mark(cs_solver* h, hipEvent_t* slot) {
   if (*slot) (void)hipEventRecord(*slot, h->st);
 }
 
+// a per-pass timing point, recorded when pass_timing is at least `level`
+// (1: broad- and narrow-phase kernels, 2: phase boundaries)
+void mark_pass(cs_solver* h, hipEvent_t* slot, int level) {
+  if (h->P.pass_timing >= level) mark(h, slot);
+  else *slot = nullptr;
+}
+
 float elapsed(hipEvent_t a, hipEvent_t b) {
   float ms = 0.f;
   if (a && b && hipEventElapsedTime(&ms, a, b) == hipSuccess) return ms;
@@ -275,7 +282,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   swap_ptr(S.hp, S.hp_prev);
   const PrevPass prev = h->prev;
   h->prev.valid = false;
-  mark(h, &T.e0);
+  mark_pass(h, &T.e0, 2);
   // the dirty-range marks of the previous pass are cleared first (their count
   // is zeroed below)
   if (S.ne >= 2) launch_clear_ranges(h->st, S);
@@ -312,9 +319,9 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     if (kind == CS_PASS_PROXIMITY) HIPCHK(h, build_bvh(h->st, S));
     HIPCHK(h, refit_bvh(h->st, S));
     for (;;) {  // traversal; re-run with a larger stream if it overflowed
-      mark(h, &T.t0);
+      mark_pass(h, &T.t0, 1);
       launch_traverse(h->st, S, dual ? S.pairs_seed : S.pairs, S.pair_cap, dual ? S.pbox : nullptr);
-      mark(h, &T.t1);
+      mark_pass(h, &T.t1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
       if (rc) return rc;
@@ -342,11 +349,11 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       launch_split_seed(h->st, S, npairs);
       stream_dev = S.counters + CNT_PPAIRS;
     }
-    mark(h, &T.e1);
+    mark_pass(h, &T.e1, 2);
     for (;;) {
-      mark(h, &T.n0);
+      mark_pass(h, &T.n0, 1);
       launch_narrow(h->st, S, NP, stream, npairs, stream_dev, S.pair_cap);
-      mark(h, &T.n1);
+      mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
       if (rc) return rc;
@@ -369,12 +376,12 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // (the incremental scheme below); no narrow-phase result carries over
     // (the proximity pass ran another test), so every pair is tested.
     launch_mark_dirty(h->st, S, vcount_prev, 0);
-    mark(h, &T.t0);
+    mark_pass(h, &T.t0, 1);
     if (exact_counts) launch_query_dirty(h->st, S, 0);  // seed boxes: count only
     launch_dirty_element_boxes(h->st, S, P.dt);
     launch_grow_dirty(h->st, S);
     launch_query_dirty(h->st, S, 1);
-    mark(h, &T.t1);
+    mark_pass(h, &T.t1, 1);
     launch_carry_pairs_from(h->st, S, S.pairs_seed, seed.npairs);
     for (int attempt = 0;; ++attempt) {
       int rc = read_counters(h);
@@ -403,12 +410,12 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
       HIPCHK(h, hipMemcpyAsync(S.pairs + carried_pairs, S.qpairs, sizeof(int2) * (size_t)nq,
                                hipMemcpyDeviceToDevice, h->st));
     cand = exact_counts ? seed.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64) : ~0ull;
-    mark(h, &T.e1);
+    mark_pass(h, &T.e1, 2);
     narrow_n = npairs;
     for (;;) {
-      mark(h, &T.n0);
+      mark_pass(h, &T.n0, 1);
       launch_narrow(h->st, S, NP, S.pairs, npairs, nullptr, 0);
-      mark(h, &T.n1);
+      mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
       if (rc) return rc;
@@ -428,7 +435,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // narrow phase only appends records and performs idempotent zone
     // merges, so it can be run again).
     launch_mark_dirty(h->st, S, vcount_prev, prev.nzone);
-    mark(h, &T.t0);
+    mark_pass(h, &T.t0, 1);
     // previous boxes: count only.  The count only feeds the exact candidate
     // total of the pass (a diagnostic), so it runs when the caller asks for
     // the parity taps; otherwise the candidate total is reported as -1.
@@ -436,13 +443,13 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     launch_dirty_element_boxes(h->st, S, P.dt);
     launch_grow_dirty(h->st, S);                     // leaves exact, ancestors grown
     launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
-    mark(h, &T.t1);
+    mark_pass(h, &T.t1, 1);
     // the candidate pair stream itself only feeds the exact pair count of the
     // pass; it is carried when the caller asks for exact counts
     launch_carry(h->st, S, exact_counts ? prev.npairs : 0, prev.nhits, prev.nhp);
     HIPCHK(h, hipMemcpyAsync(S.counters + CNT_SNAP, S.counters, sizeof(int) * 4,
                              hipMemcpyDeviceToDevice, h->st));
-    mark(h, &T.e1);
+    mark_pass(h, &T.e1, 2);
     // impact-zone merges of the carried hit pairs (union order does not
     // change the partition); their count is read on the device
     if (kind == CS_PASS_ZONE) launch_hp_unions(h->st, S, prev.nhp, 1);
@@ -450,16 +457,16 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // handled after the count is known
     const int guess = std::min(S.qpair_cap, std::max(4096, 2 * prev.nq + 1024));
     for (int attempt = 0;; ++attempt) {
-      mark(h, &T.n0);
+      mark_pass(h, &T.n0, 1);
       launch_narrow(h->st, S, NP, S.qpairs, guess, S.counters + CNT_QPAIRS, S.qpair_cap);
-      mark(h, &T.n1);
+      mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
       if (rc) return rc;
       nq = h->h_counters[CNT_QPAIRS];
       if (nq > guess && nq <= S.qpair_cap) {  // the rest of the stream
         launch_narrow(h->st, S, NP, S.qpairs + guess, nq - guess, nullptr, 0);
-        mark(h, &T.n1);
+        mark_pass(h, &T.n1, 1);
         HIPCHK(h, hipGetLastError());
         if ((rc = read_counters(h))) return rc;
       }
@@ -502,7 +509,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     cand = exact_counts && prev.cand != ~0ull ? prev.cand - c64(h, CNT_OLD64) + c64(h, CNT_NEW64)
                                               : ~0ull;
   }
-  mark(h, &T.e2);
+  mark_pass(h, &T.e2, 2);
   HIPCHK(h, apply_hits(h->st, S, nhits, vcount_cur));
   int nzone = 0;
   if (kind == CS_PASS_ZONE) {
@@ -511,7 +518,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, nzone));
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
-  mark(h, &T.e3);
+  mark_pass(h, &T.e3, 2);
   cs_pass_info pi{};
   pi.kind = kind;
   pi.index = index;
@@ -572,6 +579,7 @@ void cs_default_params(cs_params* p) {
   p->max_strain_sweeps = 10;
   p->log_pairs = 0;
   p->device = 0;
+  p->pass_timing = 2;
 }
 
 int cs_create(cs_solver** out, const cs_params* p) {

@@ -66,11 +66,16 @@ __global__ void k_uf_init(int32_t* __restrict__ uf, int nv) {
   if (v < nv) uf[v] = v;
 }
 
-__global__ void k_hit_keys(const hit_rec* __restrict__ hits, int n, uint64_t* __restrict__ keys,
-                           uint32_t* __restrict__ idx) {
+// Sort key of a hit: (lo, hi, sub-test) packed as tightly as the element
+// count allows (hi in hb bits, the sub-test, at most 15, in 4 bits), which
+// keeps the order of the record key and needs fewer digit passes.
+__global__ void k_hit_keys(const hit_rec* __restrict__ hits, int n, int hb,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ idx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  keys[i] = hits[i].key;
+  const uint64_t k = hits[i].key;
+  const uint64_t lo = k >> 36, hi = (k >> 8) & 0xFFFFFFFull, sub = k & 255u;
+  keys[i] = (((lo << hb) | hi) << 4) | sub;
   idx[i] = (uint32_t)i;
 }
 
@@ -139,9 +144,11 @@ static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rou
 hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount) {
   if (nhits <= 0) return hipSuccess;
   const int nb = (nhits + 255) / 256;
-  hipLaunchKernelGGL(k_hit_keys, dim3(nb), dim3(256), 0, st, S.hits, nhits, S.hkey, S.hidx);
+  int hb = 1;  // bits of an element id
+  while (hb < 32 && ((uint64_t)S.ne >> hb) != 0) ++hb;
+  hipLaunchKernelGGL(k_hit_keys, dim3(nb), dim3(256), 0, st, S.hits, nhits, hb, S.hkey, S.hidx);
   bool in_tmp = false;
-  const int kbits = bits_for(hit_key((uint32_t)S.ne, (uint32_t)S.ne, 255u));
+  const int kbits = bits_for(((((uint64_t)S.ne << hb) | (uint64_t)S.ne) << 4) | 15u);
   CS_HIP_RET(radix_sort_u64(st, S.hkey, S.hidx, S.hkey_tmp, S.hidx_tmp, S.sort_hist, S.scan_part, nhits, kbits,
                             &in_tmp));
   const uint32_t* sidx = in_tmp ? S.hidx_tmp : S.hidx;
@@ -212,6 +219,18 @@ __device__ inline void lane_total(double (*part)[9], int nout, double* tot) {
     for (int k = 0; k < nout; ++k) tot[k] = tot[k] + part[l][k];
 }
 
+// Members: lane l takes members l, l + 64, ... of its zone in every phase.
+// The first ZONE_CACHE members' positions, velocities, ids and fixed flags
+// are kept in LDS by the lane that loaded them, so only the first phase
+// reads them from memory; later members are read again.
+constexpr int ZONE_CACHE = 512;
+
+struct ZoneMember {
+  dvec4 o, a;  // x_old, avgVel
+  int p;       // vertex id
+  bool fixed;
+};
+
 __global__ void __launch_bounds__(ZONE_LANES)
     k_zone_update(const uint32_t* __restrict__ zkey, const uint32_t* __restrict__ zval,
                   const int32_t* __restrict__ zhead, const int32_t* __restrict__ zend,
@@ -219,6 +238,8 @@ __global__ void __launch_bounds__(ZONE_LANES)
                   const uint8_t* __restrict__ fixed, double m, double dt,
                   int* __restrict__ counters) {
   __shared__ double part[ZONE_LANES][9];
+  __shared__ double cpos[ZONE_CACHE][3], cvel[ZONE_CACHE][3];
+  __shared__ int cid[ZONE_CACHE];  // vertex id, -1 - id when fixed
   int* err = counters + CNT_ERR;
   const int l = threadIdx.x;
   const int nz = counters[CNT_ZONES];
@@ -226,26 +247,54 @@ __global__ void __launch_bounds__(ZONE_LANES)
     const int b = zhead[z];
     const int e = zend[zkey[b]];
     const int n = e - b;
-    // x_cm, v_cm
-    // (member loads ZU at a time, summed in member order)
+    // member j (b <= j < e), from the cache when it holds it
+    auto member = [&](int j) -> ZoneMember {
+      ZoneMember r;
+      const int c = j - b;
+      if (c < ZONE_CACHE) {
+        r.o = dvec4{cpos[c][0], cpos[c][1], cpos[c][2], 0.0};
+        r.a = dvec4{cvel[c][0], cvel[c][1], cvel[c][2], 0.0};
+        const int id = cid[c];
+        r.fixed = id < 0;
+        r.p = id < 0 ? -1 - id : id;
+      } else {
+        r.p = (int)zval[j];
+        r.o = x_old[r.p];
+        r.a = av[r.p];
+        r.fixed = fixed[r.p] != 0;
+      }
+      return r;
+    };
+    // x_cm, v_cm (member loads ZU at a time, summed in member order)
     double t[9];
     for (int k = 0; k < 6; ++k) t[k] = 0.0;
     for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
       dvec4 o[ZU], a[ZU];
+      int pv[ZU];
+      uint8_t fx[ZU];
 #pragma unroll
       for (int u = 0; u < ZU; ++u) {
         const int j = j0 + u * ZONE_LANES;
         if (j < e) {
           const int p = zval[j];
+          pv[u] = p;
           o[u] = x_old[p];
           a[u] = av[p];
+          fx[u] = fixed[p];
         }
       }
 #pragma unroll
       for (int u = 0; u < ZU; ++u) {
-        if (j0 + u * ZONE_LANES >= e) break;
+        const int j = j0 + u * ZONE_LANES;
+        if (j >= e) break;
         t[0] = o[u].x + t[0]; t[1] = o[u].y + t[1]; t[2] = o[u].z + t[2];
         t[3] = a[u].x + t[3]; t[4] = a[u].y + t[4]; t[5] = a[u].z + t[5];
+        const int c = j - b;
+        if (c < ZONE_CACHE) {
+          cpos[c][0] = o[u].x; cpos[c][1] = o[u].y; cpos[c][2] = o[u].z;
+          cvel[c][0] = a[u].x; cvel[c][1] = a[u].y; cvel[c][2] = a[u].z;
+          cid[c] = fx[u] ? -1 - pv[u] : pv[u];
+        }
       }
     }
     for (int k = 0; k < 6; ++k) part[l][k] = t[k];
@@ -255,59 +304,31 @@ __global__ void __launch_bounds__(ZONE_LANES)
     const double x_cm[3] = {tot[0] / n, tot[1] / n, tot[2] / n};
     const double v_cm[3] = {tot[3] / n, tot[4] / n, tot[5] / n};
     __syncthreads();
-    // angular momentum L = sum m dx x dv
-    for (int k = 0; k < 3; ++k) t[k] = 0.0;
-    for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
-      dvec4 o[ZU], a[ZU];
-#pragma unroll
-      for (int u = 0; u < ZU; ++u) {
-        const int j = j0 + u * ZONE_LANES;
-        if (j < e) {
-          const int p = zval[j];
-          o[u] = x_old[p];
-          a[u] = av[p];
+    // angular momentum L = sum m dx x dv and inertia tensor
+    // I = sum m (|dx|^2 delta - dx dx^T), in one sweep over the members
+    double tl[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 9; ++k) t[k] = 0.0;
+    for (int j = b + l; j < e; j += ZONE_LANES) {
+      const ZoneMember r = member(j);
+      const double xo[3] = {r.o.x, r.o.y, r.o.z}, vv[3] = {r.a.x, r.a.y, r.a.z};
+      double dx[3], dv[3], Li[3];
+      sub3(xo, x_cm, dx);
+      sub3(vv, v_cm, dv);
+      cross3(dx, dv, Li);
+      for (int k = 0; k < 3; ++k) tl[k] = m * Li[k] + tl[k];
+      const double mag_dx = mag3(dx);
+      for (int rr = 0; rr < 3; ++rr)
+        for (int c = 0; c < 3; ++c) {
+          double tm = -dx[rr] * dx[c];
+          if (rr == c) tm += mag_dx * mag_dx;
+          t[3 * rr + c] = tm * m + t[3 * rr + c];
         }
-      }
-#pragma unroll
-      for (int u = 0; u < ZU; ++u) {
-        if (j0 + u * ZONE_LANES >= e) break;
-        const double xo[3] = {o[u].x, o[u].y, o[u].z}, vv[3] = {a[u].x, a[u].y, a[u].z};
-        double dx[3], dv[3], Li[3];
-        sub3(xo, x_cm, dx);
-        sub3(vv, v_cm, dv);
-        cross3(dx, dv, Li);
-        for (int k = 0; k < 3; ++k) t[k] = m * Li[k] + t[k];
-      }
     }
-    for (int k = 0; k < 3; ++k) part[l][k] = t[k];
+    for (int k = 0; k < 3; ++k) part[l][k] = tl[k];
     __syncthreads();
     double L[3];
     lane_total(part, 3, L);
     __syncthreads();
-    // inertia tensor I = sum m (|dx|^2 delta - dx dx^T)
-    for (int k = 0; k < 9; ++k) t[k] = 0.0;
-    for (int j0 = b + l; j0 < e; j0 += ZU * ZONE_LANES) {
-      dvec4 o[ZU];
-#pragma unroll
-      for (int u = 0; u < ZU; ++u) {
-        const int j = j0 + u * ZONE_LANES;
-        if (j < e) o[u] = x_old[zval[j]];
-      }
-#pragma unroll
-      for (int u = 0; u < ZU; ++u) {
-        if (j0 + u * ZONE_LANES >= e) break;
-        const double xo[3] = {o[u].x, o[u].y, o[u].z};
-        double dx[3];
-        sub3(xo, x_cm, dx);
-        const double mag_dx = mag3(dx);
-        for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c) {
-            double tm = -dx[r] * dx[c];
-            if (r == c) tm += mag_dx * mag_dx;
-            t[3 * r + c] = tm * m + t[3 * r + c];
-          }
-      }
-    }
     for (int k = 0; k < 9; ++k) part[l][k] = t[k];
     __syncthreads();
     double Iv[9];
@@ -329,10 +350,9 @@ __global__ void __launch_bounds__(ZONE_LANES)
     const double cw = cs_det_cos(dt * mag_w);
     const double sw = mag_w < CS_ROUND_EPS ? 0.0 : cs_det_sin(dt * mag_w) / mag_w;
     for (int j = b + l; j < e; j += ZONE_LANES) {
-      const int p = zval[j];
-      if (fixed[p]) continue;
-      const dvec4 o = x_old[p];
-      const double xo[3] = {o.x, o.y, o.z};
+      const ZoneMember r = member(j);
+      if (r.fixed) continue;
+      const double xo[3] = {r.o.x, r.o.y, r.o.z};
       double dx[3], xF[3], xR[3], wxR[3], tmpV[3], x_new[3], an[3];
       sub3(xo, x_cm, dx);
       if (mag_w < CS_ROUND_EPS) {
@@ -353,8 +373,9 @@ __global__ void __launch_bounds__(ZONE_LANES)
         an[k] = (x_new[k] - xo[k]) / dt;
       }
       if (isnan(an[0]) || isnan(an[1]) || isnan(an[2])) atomicOr(err, CS_DEVERR_NONFINITE);
-      av[p] = dvec4{an[0], an[1], an[2], 0.0};
+      av[r.p] = dvec4{an[0], an[1], an[2], 0.0};
     }
+    __syncthreads();  // the cache is refilled by the next zone
   }
 }
 

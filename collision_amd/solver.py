@@ -91,6 +91,8 @@ class CollisionSolver3d:
     def setPointMass(self, v): self.params.mass = v; self._push()
     def getPointMass(self): return self.params.mass
     def setRestitutionCoef(self, v): self.params.restitution = v; self._push()
+    # per-pass timing points (not in the reference): 2 phases and kernels, 1 kernels, 0 none
+    def setPassTiming(self, level): self.params.pass_timing = level; self._push()
     def getRestitutionCoef(self): return self.params.restitution
 
     def setDomainBoundary(self, L, U):

@@ -57,10 +57,15 @@ typedef struct cs_params {
     int32_t log_pairs;          /* record hit pairs of every pass (parity taps) */
     int32_t device;             /* HIP device ordinal */
     int32_t force_full_passes;  /* 1: every pass re-runs the whole broad and narrow
-                                   phase (0: passes after the first CCD pass of a
-                                   step only redo pairs whose vertices changed;
-                                   results are identical either way) */
-    int32_t reserved[2];
+                                   phase (0: the first CCD pass reuses the proximity
+                                   pass's tree walk, later passes only redo pairs
+                                   whose vertices changed; results are identical
+                                   either way) */
+    int32_t pass_timing;        /* per-pass times in cs_pass_info: 2 phase and kernel
+                                   times (default), 1 broad- and narrow-phase kernel
+                                   times only, 0 none; each timing point is a marker
+                                   on the stream, which costs GPU time */
+    int32_t reserved;
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
