@@ -378,8 +378,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     launch_mark_dirty(h->st, S, vcount_prev, 0);
     mark_pass(h, &T.t0, 1);
     if (exact_counts) launch_query_dirty(h->st, S, 0);  // seed boxes: count only
-    launch_dirty_element_boxes(h->st, S, P.dt);
-    launch_grow_dirty(h->st, S);
+    launch_grow_dirty(h->st, S, P.dt);  // new boxes; leaves exact, ancestors grown
     launch_query_dirty(h->st, S, 1);
     mark_pass(h, &T.t1, 1);
     launch_carry_pairs_from(h->st, S, S.pairs_seed, seed.npairs);
@@ -440,8 +439,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     // total of the pass (a diagnostic), so it runs when the caller asks for
     // the parity taps; otherwise the candidate total is reported as -1.
     if (exact_counts) launch_query_dirty(h->st, S, 0);
-    launch_dirty_element_boxes(h->st, S, P.dt);
-    launch_grow_dirty(h->st, S);                     // leaves exact, ancestors grown
+    launch_grow_dirty(h->st, S, P.dt);               // new boxes; leaves exact, ancestors grown
     launch_query_dirty(h->st, S, 1);                 // current boxes: count + emit
     mark_pass(h, &T.t1, 1);
     // the candidate pair stream itself only feeds the exact pair count of the
@@ -514,8 +512,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   int nzone = 0;
   if (kind == CS_PASS_ZONE) {
     nzone = h->h_counters[CNT_ZLIST];
-    HIPCHK(h, hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), h->st));
-    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, nzone));
+    HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, nzone));  // (resets the zone count)
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark_pass(h, &T.e3, 2);

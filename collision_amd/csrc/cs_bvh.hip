@@ -653,41 +653,6 @@ __global__ void __launch_bounds__(256) k_mark_dirty(const int32_t* __restrict__ 
   }
 }
 
-__global__ void k_dirty_boxes(const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
-                              const dvec4* __restrict__ av, double dt,
-                              const int32_t* __restrict__ dlist, const int* __restrict__ counters,
-                              dbox* __restrict__ ebox) {
-  const int nd = counters[CNT_DIRTY];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
-  const int e = dlist[k];
-  const int4 t = ev[e];
-  const int np = (t.z >= 0) ? 3 : 2;
-  const int vid[3] = {t.x, t.y, t.z};
-  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
-  double hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-  for (int i = 0; i < np; ++i) {
-    const dvec4 xo = x_old[vid[i]];
-    const dvec4 a = av[vid[i]];
-    const double o[3] = {xo.x, xo.y, xo.z};
-    const double nn[3] = {xo.x + a.x * dt, xo.y + a.y * dt, xo.z + a.z * dt};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      hi[d] = smax(hi[d], o[d]);
-      hi[d] = smax(hi[d], nn[d]);
-      lo[d] = smin(lo[d], o[d]);
-      lo[d] = smin(lo[d], nn[d]);
-    }
-  }
-  dbox b;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    b.lo[d] = lo[d] - CS_BOX_EPS;
-    b.hi[d] = hi[d] + CS_BOX_EPS;
-  }
-  ebox[e] = b;
-  }
-}
-
 // Incremental refit by growth.  Between full refits only dirty elements
 // change their boxes; each dirty leaf gets its exact new FP32 box and then
 // grows its ancestors, stopping at the first one that already contains it.
@@ -705,17 +670,47 @@ __device__ inline void atomic_max_f32(float* a, float v) {
   else atomicMin(reinterpret_cast<unsigned int*>(a), __float_as_uint(v));
 }
 
-__global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebox,
+__global__ void __launch_bounds__(256) k_grow_dirty(const int4* __restrict__ ev,
+                                                    const dvec4* __restrict__ x_old,
+                                                    const dvec4* __restrict__ av, double dt,
                                                     const int32_t* __restrict__ dlist,
                                                     const int* __restrict__ counters,
+                                                    dbox* __restrict__ ebox,
                                                     const int32_t* __restrict__ enode,
                                                     bvh_node* nodes,
-                                                    const int32_t* __restrict__ parent) {
+                                                    const int32_t* __restrict__ parent, int grow) {
   const int nd = counters[CNT_DIRTY];
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
     const int e = dlist[k];
+    // the element's new swept box
+    const int4 t = ev[e];
+    const int np = (t.z >= 0) ? 3 : 2;
+    const int vid[3] = {t.x, t.y, t.z};
+    double blo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL};
+    double bhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int i = 0; i < np; ++i) {
+      const dvec4 xo = x_old[vid[i]];
+      const dvec4 a = av[vid[i]];
+      const double o[3] = {xo.x, xo.y, xo.z};
+      const double nn[3] = {xo.x + a.x * dt, xo.y + a.y * dt, xo.z + a.z * dt};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        bhi[d] = smax(bhi[d], o[d]);
+        bhi[d] = smax(bhi[d], nn[d]);
+        blo[d] = smin(blo[d], o[d]);
+        blo[d] = smin(blo[d], nn[d]);
+      }
+    }
+    dbox b;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = blo[d] - CS_BOX_EPS;
+      b.hi[d] = bhi[d] + CS_BOX_EPS;
+    }
+    ebox[e] = b;
+    if (!grow) continue;
+    // the leaf exactly, its ancestors grown
     const int leaf = enode[e];
-    const dbox b = ebox[e];
     float lo[3], hi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -726,8 +721,8 @@ __global__ void __launch_bounds__(256) k_grow_dirty(const dbox* __restrict__ ebo
     }
     for (int node = parent[leaf]; node >= 0; node = parent[node]) {
       const bvh_node c = nodes[node];
-      if (c.lo[0] <= lo[0] && c.lo[1] <= lo[1] && c.lo[2] <= lo[2] && c.hi[0] >= hi[0] &&
-          c.hi[1] >= hi[1] && c.hi[2] >= hi[2])
+      if ((c.lo[0] <= lo[0]) & (c.lo[1] <= lo[1]) & (c.lo[2] <= lo[2]) & (c.hi[0] >= hi[0]) &
+          (c.hi[1] >= hi[1]) & (c.hi[2] >= hi[2]))
         break;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -1091,16 +1086,12 @@ void launch_zero_counters(hipStream_t st, DevState& S, uint32_t mask) {
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(32), 0, st, S.counters, mask);
 }
 
-void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt) {
-  // grid sized for every element; lanes beyond the dirty count exit
-  hipLaunchKernelGGL(k_dirty_boxes, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st, S.ev, S.x_old,
-                     S.av, dt, S.dlist, S.counters, S.ebox);
-}
-
-void launch_grow_dirty(hipStream_t st, DevState& S) {
-  if (S.ne < 2) return;
+void launch_grow_dirty(hipStream_t st, DevState& S, double dt) {
+  if (S.ne == 0) return;
+  // grid for a typical dirty list; longer lists are walked grid-stride
   hipLaunchKernelGGL(k_grow_dirty, dim3(std::min((S.ne + 255) / 256, 512)), dim3(256), 0, st,
-                     S.ebox, S.dlist, S.counters, S.leaf_of, S.nodes, S.parent);
+                     S.ev, S.x_old, S.av, dt, S.dlist, S.counters, S.ebox, S.leaf_of, S.nodes,
+                     S.parent, S.ne >= 2 ? 1 : 0);
 }
 
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {

@@ -187,7 +187,6 @@ hipError_t radix_sort_u64(hipStream_t st, uint64_t* keys, uint32_t* vals, uint64
 
 // broad phase
 void launch_element_boxes(hipStream_t st, const DevState& S, int moving, double dt, dbox* out);
-void launch_dirty_element_boxes(hipStream_t st, const DevState& S, double dt);
 hipError_t build_bvh(hipStream_t st, DevState& S);  // also finds the refit top nodes
 hipError_t refit_bvh(hipStream_t st, DevState& S);
 // candidate pairs of the current boxes (ebox) into out; with pbox (static
@@ -208,7 +207,8 @@ void launch_zero_counters(hipStream_t st, DevState& S, uint32_t mask);
 constexpr int QW_RANGE_SHIFT = 3;
 constexpr int QW_RANGE = 1 << QW_RANGE_SHIFT;  // pre-order nodes per dirty-query range
 void launch_query_dirty(hipStream_t st, DevState& S, int emit);
-void launch_grow_dirty(hipStream_t st, DevState& S);
+// the dirty elements' new swept boxes, their leaves exact and ancestors grown
+void launch_grow_dirty(hipStream_t st, DevState& S, double dt);
 void launch_carry(hipStream_t st, DevState& S, int npairs_prev, int nhits_prev, int nhp_prev);
 // pairs of src[0..n) with no dirty element into S.pairs (count CNT_PAIRS)
 void launch_carry_pairs_from(hipStream_t st, DevState& S, const int2* src, int n);

@@ -177,9 +177,12 @@ __device__ inline int uf_root(const int32_t* uf, int x) {
 
 // sort keys of the zone members: (root << vbits) | vertex, so one sort
 // orders zones by root and members by ascending vertex id
+// (also resets the zone count, which k_zone_heads fills after the sort)
 __global__ void k_zone_keys(const uint32_t* __restrict__ zlist, int n, const int32_t* __restrict__ uf,
-                            int vbits, uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+                            int vbits, uint64_t* __restrict__ key, uint32_t* __restrict__ val,
+                            int* __restrict__ counters) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) counters[CNT_ZONES] = 0;
   if (i >= n) return;
   const uint32_t v = zlist[i];
   const uint32_t r = (uint32_t)uf_root(uf, (int)v);
@@ -383,12 +386,12 @@ __global__ void __launch_bounds__(ZONE_LANES)
 // the union calls list the members as they go (zone_touch); only the listed
 // vertices are sorted, not the whole mesh.
 hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nlist) {
-  if (nlist <= 0) return hipSuccess;
+  if (nlist <= 0) return hipMemsetAsync(S.counters + CNT_ZONES, 0, sizeof(int), st);  // no zones
   const int nb = (nlist + 255) / 256;
   int vbits = 1;
   while (vbits < 32 && ((uint64_t)(S.nv - 1) >> vbits) != 0) ++vbits;
   hipLaunchKernelGGL(k_zone_keys, dim3(nb), dim3(256), 0, st, S.zlist, nlist, S.uf, vbits,
-                     S.zkey64, S.zval);
+                     S.zkey64, S.zval, S.counters);
   bool in_tmp = false;
   CS_HIP_RET(radix_sort_u64(st, S.zkey64, S.zval, S.zkey64_tmp, S.zval_tmp, S.sort_hist,
                             S.scan_part, nlist, bits_for(((uint64_t)(S.nv - 1) << vbits) | (uint64_t)(S.nv - 1)),
