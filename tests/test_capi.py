@@ -31,6 +31,8 @@ def test_default_params_are_the_reference_defaults():
     assert (p.dt, p.thickness, p.round_eps, p.spring_k, p.mass, p.friction, p.restitution) == \
         (1e-3, 1e-4, 1e-6, 1000.0, 0.01, 0.02, 0.0)
     assert p.max_ccd_passes == 5 and p.max_strain_sweeps == 10
+    # library switches: incremental passes on, every per-pass time recorded
+    assert p.force_full_passes == 0 and p.pass_timing == 2
 
 
 def test_create_without_device_is_a_status_not_a_crash():
