@@ -125,10 +125,18 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
     /* marks R or more levels ahead (one sweep in flight): per-slot bytes,
        taken by the first sweep's scan of that level */
     uint8_t* far=calloc(nsl,1);
+    /* levels the first sweep must visit: a pre-check hit or a far mark */
+    uint8_t* preb=calloc(L,1);
+    for(int s=0;s<nsl;s++) if(pre[s]) preb[lev[slot_elem[s]]]=1;
     uint8_t ring[R][CAPL]; memset(ring,0,sizeof ring); int owner[R]; for(int i=0;i<R;i++) owner[i]=-1;
     long edges[16]={0};
     int tend = L + (K-1)*dl;
-    for (int t=0; t<tend; t++){
+    /* as the kernel: start at the first such level; after an iteration, when
+       no mark is pending (every mark is consumed within dl iterations of the
+       iteration that made it), jump to the next such level */
+    int last_mark = -(1<<29);
+    int t = 0; while (t < L && !preb[t]) t++;
+    for (; t<tend; ){
       /* scan */
       int nw=0; static int wk_s[16*CAPL], wk_k[16*CAPL], wk_f[16*CAPL];
       for (int k=0;k<K;k++){ int l=t-k*dl; if(l<0||l>=L) continue; int n=lp[l+1]-lp[l];
@@ -157,18 +165,26 @@ static int pipelined(d4* av, int maxit, int* last_edges, long* degen_sum, long* 
           if (ts==s || (wrap && k==K-1)) { if (wrote) { d4 c=a[p]; c.w=s; cp[s][p]=c; } continue; }
           d4 c=a[p]; c.w=origin; cp[ts][tpos]=c;
           if (ts==(int)origin) continue;
-          if (!wrap && tl - l >= R){ if (K!=1){ printf("far mark with %d sweeps in flight\n",K); exit(1);} far[ts] |= 1<<tpos; far_marks++; continue; }
+          if (!wrap && tl - l >= R){ if (K!=1){ printf("far mark with %d sweeps in flight\n",K); exit(1);} far[ts] |= 1<<tpos; preb[tl]=1; far_marks++; continue; }
           int ri=tl&(R-1); if(owner[ri]!=-1 && owner[ri]!=tl){ printf("ring alias mark t=%d tl=%d owner=%d\n",t,tl,owner[ri]); exit(1);} owner[ri]=tl;
           ring[ri][ts-lp[tl]] |= 1<<tpos;
+          last_mark = t;
         }
-        if (ed && k<K-1){ int ri=l&(R-1); if(owner[ri]!=-1 && owner[ri]!=l){ printf("ring alias fired\n"); exit(1);} owner[ri]=l; ring[ri][s-lp[l]] |= 8; }
+        if (ed && k<K-1){ int ri=l&(R-1); if(owner[ri]!=-1 && owner[ri]!=l){ printf("ring alias fired\n"); exit(1);} owner[ri]=l; ring[ri][s-lp[l]] |= 8; last_mark = t; }
       }
+      int tn = t + 1;
+      /* planted mistakes: 2 skips one iteration early, 3 skips with marks pending */
+      if (mistake == 3 || t >= last_mark + dl - (mistake == 2 ? 1 : 0)) {
+        tn = t + 1; while (tn < L && !preb[tn]) tn++;
+        if (tn >= L) break;
+      }
+      t = tn;
     }
     /* writeback: each vertex's value from its last visit's copy */
     for(int v=0;v<nv;v++) if(la_slot[v]>=0){ d4 c=cp[la_slot[v]][la_pos[v]]; av[v].x=c.x; av[v].y=c.y; av[v].z=c.z; }
     int n=0; while(n<K && edges[n]>0) n++; int niter = n<K ? n+1 : K;
     total_sweeps += niter; *last_edges = edges[niter-1];
-    free(pre); free(far);
+    free(pre); free(far); free(preb);
     if (niter < K || edges[K-1]==0) break;
     k0 += K;
     for(int s=0;s<nsl;s++){ int e=slot_elem[s]; int np=T[e][2]>=0?3:2; for(int k=0;k<np;k++){ cp[s][k]=av[T[e][k]]; cp[s][k].w=-1; } }

@@ -6,8 +6,11 @@ a span-bounded schedule, and evaluates only the slots whose result can
 change.  tests/cpp/strain_pipeline_model.c restates that scheme on the CPU and
 compares it bit for bit with the sequential loop (dcollid.cpp:583-596) on the
 meshes below (the fan disc's schedule leaves one sweep in flight and flags
-visits 64 or more levels ahead outside the ring); two planted mistakes (sweeps run closer than the schedule
-allows; tokens of non-writing visits dropped) must be caught.  The GPU kernel
+visits 64 or more levels ahead outside the ring).  Like the kernel, the model
+jumps ahead to the next level holding a pre-check hit or a far mark once no
+mark is pending.  Four planted mistakes (sweeps run closer than the schedule
+allows; tokens of non-writing visits dropped; levels skipped one iteration
+early; levels skipped with marks pending) must be caught.  The GPU kernel
 itself is compared with the oracle by the GPU parity tests.
 """
 import os
@@ -92,9 +95,14 @@ def test_pipelined_sweeps_equal_sequential_loop(model, mesh, case):
         assert "far=0 " not in out, out
 
 
-@pytest.mark.parametrize("mistake", [("-1", "0"), ("0", "1")])
+# (mesh, lag offset, mistake): sweeps one level too close; tokens of
+# non-writing visits dropped; levels skipped one iteration before every mark
+# is consumed; levels skipped with marks pending
+@pytest.mark.parametrize("mistake", [("cloth_strings_boxes", "-1", "0"),
+                                     ("cloth_strings_boxes", "0", "1"),
+                                     ("string_on_box", "0", "2"),
+                                     ("cloth_strings_boxes", "0", "3")])
 def test_planted_mistakes_are_caught(model, mistake):
-    lag, kind = mistake
-    rc, out = run(model, "cloth_strings_boxes", MESHES["cloth_strings_boxes"](),
-                  0.004, 300.0, 10, 1024, 1, lag, kind)
+    mesh, lag, kind = mistake
+    rc, out = run(model, mesh, MESHES[mesh](), 0.004, 300.0, 10, 1024, 1, lag, kind)
     assert rc != 0, out
