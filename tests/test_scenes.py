@@ -52,3 +52,19 @@ def test_series_functions_match_numpy():
     np.testing.assert_allclose(c, np.cos(2 * np.pi * t), atol=2e-15)
     n = S.normal(3, 400000, 1.0)
     assert abs(n.mean()) < 0.01 and abs(n.std() - 1) < 0.01
+
+
+def test_fan_disc_has_one_high_valence_vertex():
+    """The fan disc used by the one-sweep-in-flight strain case: nfan triangles
+    around vertex 0, (rings - 1) bands of 2 * nfan triangles, a static box, and
+    the same arrays on every call."""
+    a, b = S.fan_disc_on_box(), S.fan_disc_on_box()
+    assert digest(a) == digest(b)
+    nfan, rings = 96, 3
+    movable = ~a.fixed.astype(bool)[a.tris].any(axis=1)
+    assert movable.sum() == nfan + 2 * nfan * (rings - 1)
+    valence = np.bincount(a.tris[movable].ravel(), minlength=a.nv)
+    assert valence[0] == nfan and np.sort(valence)[-2] <= 6
+    # the disc lies above the box, so the first step has no contact
+    box_top = a.x[a.fixed.astype(bool), 2].max()
+    assert a.x[~a.fixed.astype(bool), 2].min() > box_top
