@@ -52,6 +52,10 @@ class CollisionSolver3d {
   double getPointMass() const { return p_.mass; }
   void setRestitutionCoef(double v) { p_.restitution = v; push(); }
   double getRestitutionCoef() const { return p_.restitution; }
+  // not in the reference: per-pass timing points (2 phases and kernels, 1 kernels, 0 none);
+  // each is a marker on the stream and costs GPU time
+  void setPassTiming(int level) { p_.pass_timing = level; push(); }
+  int getPassTiming() const { return p_.pass_timing; }
   void setDomainBoundary(const double* L, const double* U) {
     check(cs_set_domain_boundary(h_, L, U));
     for (int i = 0; i < 3; ++i) {
