@@ -38,14 +38,16 @@ class Params(C.Structure):
                 ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
                 ("log_pairs", C.c_int32), ("device", C.c_int32),
                 ("force_full_passes", C.c_int32), ("pass_timing", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("record_impulses", C.c_int32)]
 
 
 class MeshView(C.Structure):
     _fields_ = [("nv", C.c_int32), ("nt", C.c_int32), ("nb", C.c_int32),
                 ("tris", C.POINTER(C.c_int32)), ("tri_surf", C.POINTER(C.c_int32)),
                 ("tri_len0", C.POINTER(C.c_double)), ("bonds", C.POINTER(C.c_int32)),
-                ("bond_len0", C.POINTER(C.c_double)), ("is_fixed", C.POINTER(C.c_uint8))]
+                ("bond_len0", C.POINTER(C.c_double)), ("is_fixed", C.POINTER(C.c_uint8)),
+                ("nbody", C.c_int32), ("vert_body", C.POINTER(C.c_int32)),
+                ("body_mass", C.POINTER(C.c_double)), ("body_com", C.POINTER(C.c_double))]
 
 
 class PassInfo(C.Structure):
@@ -113,6 +115,12 @@ def lib(path: str = LIB_PATH):
         "cs_pair_log_count": ([vp], i64),
         "cs_pair_log": ([vp, P(i32), i64], C.c_int),
         "cs_zone_labels": ([vp, P(i32)], C.c_int),
+        "cs_get_bodies": ([vp, P(d), P(d)], C.c_int),
+        "cs_math_eval": ([i32, i32, P(d), P(d), i64], C.c_int),
+        "cs_last_impulses": ([vp, P(d), P(i32)], C.c_int),
+        "cs_write_vtk": ([vp, C.c_char_p], C.c_int),
+        "cs_save": ([vp, C.c_char_p], C.c_int),
+        "cs_load": ([vp, C.c_char_p], C.c_int),
         "cs_last_error": ([vp], C.c_char_p),
         "cs_version": ([], C.c_char_p),
     }

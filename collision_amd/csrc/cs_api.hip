@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,9 @@ void dfree(T*& p) {
 void free_dev(DevState& S) {
   dfree(S.x); dfree(S.x_old); dfree(S.vel); dfree(S.av);
   dfree(S.fixed); dfree(S.has); dfree(S.uf);
+  dfree(S.vbody); dfree(S.bptr); dfree(S.bmem); dfree(S.bfirst); dfree(S.bvisit);
+  dfree(S.bmass); dfree(S.bcom); dfree(S.bvel); dfree(S.bmrg); dfree(S.rg_imp); dfree(S.rg_num);
+  dfree(S.pimp); dfree(S.pnum);
   dfree(S.ev); dfree(S.eflags); dfree(S.len0); dfree(S.ebox);
   dfree(S.bounds); dfree(S.mcode); dfree(S.mid); dfree(S.mcode_tmp); dfree(S.mid_tmp);
   dfree(S.sort_hist); dfree(S.nodes); dfree(S.parent); dfree(S.nright); dfree(S.split_right);
@@ -267,11 +271,14 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   DevState& S = h->S;
   const cs_params& P = h->P;
   const int ccd = kind != CS_PASS_PROXIMITY;
-  const bool incremental = ccd && h->prev.valid && !P.force_full_passes;
+  // with movable bodies every pass changes all body points (their zones are
+  // moved rigidly in every apply), so passes are not incremental
+  const bool full_only = P.force_full_passes || S.nbody > 0;
+  const bool incremental = ccd && h->prev.valid && !full_only;
   // the first CCD pass starts from the proximity pass's swept-box pairs
   const bool seeded = kind == CS_PASS_CCD && !incremental && h->seed.valid;
   // the proximity pass walks the tree once for itself and the first CCD pass
-  const bool dual = kind == CS_PASS_PROXIMITY && !P.force_full_passes;
+  const bool dual = kind == CS_PASS_PROXIMITY && !full_only;
   const SeedPairs seed = h->seed;
   h->seed.valid = false;
   const bool exact_counts = P.log_pairs != 0;
@@ -299,7 +306,8 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
                            (1u << (CNT_VLIST + (pass_id & 1))));
   NarrowParams NP;
   NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
-  NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction;
+  NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction; NP.cr = P.restitution;
+  NP.vbody = S.vbody; NP.bmass = S.bmass;
   NP.ccd = ccd;
   NP.zone_mode = (kind == CS_PASS_ZONE);
   NP.log_pairs = P.log_pairs;
@@ -508,11 +516,13 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
                                               : ~0ull;
   }
   mark_pass(h, &T.e2, 2);
-  HIPCHK(h, apply_hits(h->st, S, nhits, vcount_cur));
+  HIPCHK(h, apply_hits(h->st, S, nhits, vcount_cur, P.record_impulses != 0 && S.pimp));
+  HIPCHK(h, apply_bodies(h->st, S, P.mass, P.dt));  // RG impulses, body zones (updateAverageVelocity)
   int nzone = 0;
   if (kind == CS_PASS_ZONE) {
     nzone = h->h_counters[CNT_ZLIST];
     HIPCHK(h, update_zones(h->st, S, P.mass, P.dt, nzone));  // (resets the zone count)
+    HIPCHK(h, update_body_zones(h->st, S, P.mass, P.dt));    // body zones hold >= 2 points
   }
   if (P.log_pairs) launch_hp_log(h->st, S, nhp, pass_id);
   mark_pass(h, &T.e3, 2);
@@ -553,6 +563,62 @@ int check_ready(cs_solver* h) {
   return CS_OK;
 }
 
+}  // namespace
+
+// checkpoint file: header, the parameters, then the arrays
+namespace {
+constexpr char kCkptMagic[8] = {'C', 'S', 'C', 'K', 'P', 'T', '0', '1'};
+struct CkptHeader {
+  char magic[8];
+  int32_t nv, nt, nb, nbody;
+  uint64_t topo;  // FNV-1a hash of the element table and the flags
+  cs_params params;
+};
+
+uint64_t topo_hash(const std::vector<int4>& ev, const std::vector<uint8_t>& ef,
+                   const std::vector<uint8_t>& fixed) {
+  uint64_t hsh = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) hsh = (hsh ^ b[i]) * 1099511628211ull;
+  };
+  mix(ev.data(), sizeof(int4) * ev.size());
+  mix(ef.data(), ef.size());
+  mix(fixed.data(), fixed.size());
+  return hsh;
+}
+
+int checkpoint_topology(cs_solver* h, uint64_t* out) {
+  DevState& S = h->S;
+  std::vector<int4> ev((size_t)S.ne);
+  std::vector<uint8_t> ef((size_t)S.ne), fixed((size_t)S.nv);
+  if (S.ne) {
+    HIPCHK(h, hipMemcpy(ev.data(), S.ev, sizeof(int4) * S.ne, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ef.data(), S.eflags, S.ne, hipMemcpyDeviceToHost));
+  }
+  HIPCHK(h, hipMemcpy(fixed.data(), S.fixed, S.nv, hipMemcpyDeviceToHost));
+  *out = topo_hash(ev, ef, fixed);
+  return CS_OK;
+}
+
+// the device arrays a checkpoint holds, in file order
+struct CkptArray {
+  void* dev;
+  size_t bytes;
+};
+std::vector<CkptArray> checkpoint_arrays(DevState& S) {
+  std::vector<CkptArray> a = {{S.x, sizeof(dvec4) * S.nv},     {S.x_old, sizeof(dvec4) * S.nv},
+                              {S.vel, sizeof(dvec4) * S.nv},   {S.av, sizeof(dvec4) * S.nv},
+                              {S.has, (size_t)S.nv}};
+  if (S.nbody > 0) {
+    a.push_back({S.bcom, sizeof(dvec4) * S.nbody});
+    a.push_back({S.bvel, sizeof(dvec4) * S.nbody});
+    a.push_back({S.bmrg, sizeof(dvec4) * S.nbody});
+    a.push_back({S.rg_imp, sizeof(dvec4) * S.nbody});
+    a.push_back({S.rg_num, sizeof(int32_t) * S.nbody});
+  }
+  return a;
+}
 }  // namespace
 
 extern "C" {
@@ -652,6 +718,21 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   std::vector<uint8_t> fixed((size_t)nv, 0);
   if (m->is_fixed) std::memcpy(fixed.data(), m->is_fixed, (size_t)nv);
   std::vector<uint8_t> used((size_t)nv, 0);
+  // movable bodies: a body is whole triangles of movable vertices of that
+  // body; no bond holds a movable vertex
+  const int nbody = m->nbody;
+  if (nbody < 0 || (nbody > 0 && (!m->vert_body || !m->body_mass || !m->body_com)))
+    return fail(h, CS_ERR_ARG, "invalid movable-body arrays");
+  std::vector<int32_t> vbody((size_t)nv, -1);
+  if (nbody > 0) {
+    for (int v = 0; v < nv; ++v) {
+      const int b = m->vert_body[v];
+      if (b < -1 || b >= nbody) return fail(h, CS_ERR_ARG, "vertex body id out of range");
+      if (b >= 0 && fixed[v]) return fail(h, CS_ERR_ARG, "a vertex cannot be static and movable");
+      vbody[v] = b;
+    }
+  }
+  auto movable = [&](int v) { return vbody[v] >= 0; };
   for (int e = 0; e < nt; ++e) {
     int v[3];
     for (int j = 0; j < 3; ++j) {
@@ -659,8 +740,12 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
       if (v[j] < 0 || v[j] >= nv) return fail(h, CS_ERR_ARG, "triangle vertex id out of range");
       used[v[j]] = 1;
     }
+    if (vbody[v[1]] != vbody[v[0]] || vbody[v[2]] != vbody[v[0]])
+      return fail(h, CS_ERR_ARG, "a triangle mixes vertices of different bodies");
     ev[e] = make_int4(v[0], v[1], v[2], m->tri_surf ? m->tri_surf[e] : 0);
-    ef[e] = EF_TRI | ((fixed[v[0]] || fixed[v[1]] || fixed[v[2]]) ? EF_RIGID : 0);
+    // isRigidBody(hse): a static or movable point (dcollid.cpp:1091-1094)
+    const bool rigid = fixed[v[0]] || fixed[v[1]] || fixed[v[2]] || movable(v[0]);
+    ef[e] = EF_TRI | (rigid ? EF_RIGID : 0);
     l0[e] = dvec4{m->tri_len0 ? m->tri_len0[3 * e] : 0.0, m->tri_len0 ? m->tri_len0[3 * e + 1] : 0.0,
                   m->tri_len0 ? m->tri_len0[3 * e + 2] : 0.0, 0.0};
   }
@@ -670,6 +755,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     if (v0 < 0 || v0 >= nv || v1 < 0 || v1 >= nv)
       return fail(h, CS_ERR_ARG, "bond vertex id out of range");
     used[v0] = used[v1] = 1;
+    if (movable(v0) || movable(v1)) return fail(h, CS_ERR_ARG, "a bond holds a movable vertex");
     ev[e] = make_int4(v0, v1, -1, -1);
     ef[e] = (fixed[v0] || fixed[v1]) ? EF_RIGID : 0;
     l0[e] = dvec4{m->bond_len0 ? m->bond_len0[b] : 0.0, 0.0, 0.0, 0.0};
@@ -843,6 +929,64 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, hipMemcpy(S.vadj, vadj.data(), sizeof(int32_t) * vadj.size(), hipMemcpyHostToDevice));
   HIPCHK(h, hipMemset(S.dpre, 0, 2 * (size_t)ne - 1));
   HIPCHK(h, hipMemset(S.rflag, 0, sizeof(int32_t) * ((2 * (size_t)ne - 1 + QW_RANGE - 1) / QW_RANGE)));
+  if (nbody > 0) {
+    // members ascending by vertex id, and in order of first appearance in
+    // element order (updateFinalForRG's visiting order)
+    std::vector<int32_t> bptr((size_t)nbody + 1, 0), bfirst((size_t)nbody + 1, 0);
+    std::vector<uint32_t> bmem;
+    std::vector<int32_t> bvisit;
+    for (int v = 0; v < nv; ++v)
+      if (vbody[v] >= 0) bptr[(size_t)vbody[v] + 1]++;
+    for (int b = 0; b < nbody; ++b) {
+      if (bptr[(size_t)b + 1] < 2) return fail(h, CS_ERR_ARG, "a movable body needs two or more vertices");
+      bptr[(size_t)b + 1] += bptr[b];
+    }
+    bmem.resize((size_t)bptr[nbody]);
+    {
+      std::vector<int32_t> fill(bptr.begin(), bptr.end() - 1);
+      for (int v = 0; v < nv; ++v)
+        if (vbody[v] >= 0) bmem[(size_t)fill[vbody[v]]++] = (uint32_t)v;
+    }
+    std::vector<std::vector<int32_t>> order((size_t)nbody);
+    std::vector<uint8_t> seen((size_t)nv, 0);
+    for (int e = 0; e < nt; ++e)
+      for (int j = 0; j < 3; ++j) {
+        const int v = m->tris[3 * e + j];
+        if (vbody[v] >= 0 && !seen[v]) {
+          seen[v] = 1;
+          order[vbody[v]].push_back(v);
+        }
+      }
+    for (int b = 0; b < nbody; ++b) {
+      bfirst[(size_t)b + 1] = bfirst[b] + (int)order[b].size();
+      bvisit.insert(bvisit.end(), order[b].begin(), order[b].end());
+    }
+    if ((int)bvisit.size() != bptr[nbody])
+      return fail(h, CS_ERR_ARG, "every movable vertex must belong to a triangle of its body");
+    std::vector<dvec4> com((size_t)nbody), zero((size_t)nbody, dvec4{0.0, 0.0, 0.0, 0.0});
+    for (int b = 0; b < nbody; ++b)
+      com[b] = dvec4{m->body_com[3 * b], m->body_com[3 * b + 1], m->body_com[3 * b + 2], 0.0};
+    S.nbody = nbody;
+    HIPCHK(h, dalloc(&S.vbody, nv)); HIPCHK(h, dalloc(&S.bptr, nbody + 1));
+    HIPCHK(h, dalloc(&S.bmem, bmem.size())); HIPCHK(h, dalloc(&S.bfirst, nbody + 1));
+    HIPCHK(h, dalloc(&S.bvisit, bvisit.size())); HIPCHK(h, dalloc(&S.bmass, nbody));
+    HIPCHK(h, dalloc(&S.bcom, nbody)); HIPCHK(h, dalloc(&S.bvel, nbody));
+    HIPCHK(h, dalloc(&S.bmrg, nbody)); HIPCHK(h, dalloc(&S.rg_imp, nbody));
+    HIPCHK(h, dalloc(&S.rg_num, nbody));
+    HIPCHK(h, hipMemcpy(S.vbody, vbody.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bptr, bptr.data(), sizeof(int32_t) * (nbody + 1), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bmem, bmem.data(), sizeof(uint32_t) * bmem.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bfirst, bfirst.data(), sizeof(int32_t) * (nbody + 1), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bvisit, bvisit.data(), sizeof(int32_t) * bvisit.size(), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bmass, m->body_mass, sizeof(double) * nbody, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bcom, com.data(), sizeof(dvec4) * nbody, hipMemcpyHostToDevice));
+    // the reference reads mrg_com before its first write (an empty vector);
+    // it starts at the body's center of mass here
+    HIPCHK(h, hipMemcpy(S.bmrg, com.data(), sizeof(dvec4) * nbody, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.bvel, zero.data(), sizeof(dvec4) * nbody, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(S.rg_imp, zero.data(), sizeof(dvec4) * nbody, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemset(S.rg_num, 0, sizeof(int32_t) * nbody));
+  }
   h->assembled = true;
   h->state_set = false;
   return CS_OK;
@@ -862,6 +1006,10 @@ int cs_set_state(cs_solver* h, const double* x, const double* vel, const uint8_t
   HIPCHK(h, hipMemset(S.av, 0, sizeof(dvec4) * S.nv));
   if (has_collsn) HIPCHK(h, hipMemcpy(S.has, has_collsn, S.nv, hipMemcpyHostToDevice));
   else HIPCHK(h, hipMemset(S.has, 0, S.nv));
+  if (S.nbody > 0) {  // initSurfaceState clears the RG accumulators (cdinit.cpp:192-197)
+    HIPCHK(h, hipMemset(S.rg_imp, 0, sizeof(dvec4) * S.nbody));
+    HIPCHK(h, hipMemset(S.rg_num, 0, sizeof(int32_t) * S.nbody));
+  }
   h->state_set = true;
   return CS_OK;
 }
@@ -890,8 +1038,8 @@ int cs_get_state(cs_solver* h, double* x, double* vel, double* avg_vel, double* 
 int cs_record_origin_position(cs_solver* h) {
   int rc = check_ready(h);
   if (rc) return rc;
-  HIPCHK(h, hipMemcpyAsync(h->S.x_old, h->S.x, sizeof(dvec4) * h->S.nv, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(h, hipMemsetAsync(h->S.has, 0, h->S.nv, h->st));
+  launch_record_origin(h->st, h->S);
+  HIPCHK(h, hipGetLastError());
   return CS_OK;
 }
 
@@ -922,14 +1070,28 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   if (S.ne >= 2) launch_clear_ranges(h->st, S);
   HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
   launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
+  if (P.record_impulses) {  // the impulses of this resolve's last pass with a hit
+    if (!S.pimp) {
+      HIPCHK(h, dalloc(&S.pimp, S.nv));
+      HIPCHK(h, dalloc(&S.pnum, S.nv));
+    }
+    HIPCHK(h, hipMemsetAsync(S.pimp, 0, sizeof(dvec4) * (size_t)S.nv, h->st));
+    HIPCHK(h, hipMemsetAsync(S.pnum, 0, sizeof(int32_t) * (size_t)S.nv, h->st));
+  }
   HIPCHK(h, hipMemsetAsync(S.zmark, 0, sizeof(int32_t) * (size_t)S.nv, h->st));
   launch_average_velocity(h->st, S, P.dt);        // computeAverageVelocity
   int pass_id = 0;
   bool any = false;
   int status = CS_OK;
+  // a non-finite average velocity ends the step before any pass, where the
+  // reference calls clean_up(ERROR) (dcollid.cpp:178-184)
+  if ((rc = read_counters(h))) return rc;
+  if (h->h_counters[CNT_ERR]) status = device_error(h);
   // detectProximity (dcollid.cpp:390-406)
-  rc = run_pass(h, CS_PASS_PROXIMITY, 0, pass_id++, &any);
-  if (rc) status = rc;
+  if (!status) {
+    rc = run_pass(h, CS_PASS_PROXIMITY, 0, pass_id++, &any);
+    if (rc) status = rc;
+  }
   // detectCollision (dcollid.cpp:430-468)
   bool is_collision = true;
   int niter = 1;
@@ -959,12 +1121,13 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_LEVELS, 0, sizeof(int) * 2, h->st));
     HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps));
     launch_final_velocity(h->st, S);
+    launch_final_bodies(h->st, S, P.dt);  // updateFinalForRG
     HIPCHK(h, hipGetLastError());
   }
   mark(h, &t_end);
   rc = read_counters(h);
   if (rc) return rc;
-  if (R.n_zone_passes) R.n_zones = h->h_counters[CNT_ZONES];
+  if (R.n_zone_passes) R.n_zones = h->h_counters[CNT_ZONES] + S.nbody;
   for (size_t i = 0; i < h->passes.size() && i < h->ptimes.size(); ++i) {
     cs_pass_info& pi = h->passes[i];
     const PassTimes& T = h->ptimes[i];
@@ -1012,6 +1175,40 @@ int cs_step(cs_solver* h, cs_step_report* report) {
 
 int cs_has_collision(cs_solver* h) { return h ? h->has_collision : 0; }
 
+int cs_get_bodies(cs_solver* h, double* com, double* com_vel) {
+  if (!h) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  DevState& S = h->S;
+  if (S.nbody == 0) return CS_OK;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  std::vector<dvec4> buf((size_t)S.nbody);
+  if (com) {
+    HIPCHK(h, hipMemcpy(buf.data(), S.bcom, sizeof(dvec4) * S.nbody, hipMemcpyDeviceToHost));
+    from_dvec4(buf, com);
+  }
+  if (com_vel) {
+    HIPCHK(h, hipMemcpy(buf.data(), S.bvel, sizeof(dvec4) * S.nbody, hipMemcpyDeviceToHost));
+    from_dvec4(buf, com_vel);
+  }
+  return CS_OK;
+}
+
+int cs_math_eval(int32_t device, int32_t fn, const double* x, double* y, int64_t n) {
+  if (!x || !y || n < 0 || fn < 0 || fn > 3) return CS_ERR_ARG;
+  if (n == 0) return CS_OK;
+  if (hipSetDevice(device) != hipSuccess) return CS_ERR_NO_DEVICE;
+  double *dx = nullptr, *dy = nullptr;
+  hipError_t e = dalloc(&dx, (size_t)n);
+  if (e == hipSuccess) e = dalloc(&dy, (size_t)n);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_math_eval(nullptr, fn, dx, dy, n);
+  if (e == hipSuccess) e = hipMemcpy(y, dy, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  dfree(dx);
+  dfree(dy);
+  return e == hipSuccess ? CS_OK : CS_ERR_HIP;
+}
+
 int cs_synchronize(cs_solver* h) {
   if (!h) return CS_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->st));
@@ -1049,6 +1246,142 @@ int cs_zone_labels(cs_solver* h, int32_t* out) {
   if (e == hipSuccess) e = hipStreamSynchronize(h->st);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(h, CS_ERR_HIP, hipGetErrorString(e));
+  return CS_OK;
+}
+
+int cs_last_impulses(cs_solver* h, double* imp, int32_t* num) {
+  if (!h) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  DevState& S = h->S;
+  if (!S.pimp) return fail(h, CS_ERR_ARG, "record_impulses was not set for the last resolve");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  if (imp) {
+    std::vector<dvec4> buf((size_t)S.nv);
+    HIPCHK(h, hipMemcpy(buf.data(), S.pimp, sizeof(dvec4) * S.nv, hipMemcpyDeviceToHost));
+    from_dvec4(buf, imp);
+  }
+  if (num) HIPCHK(h, hipMemcpy(num, S.pnum, sizeof(int32_t) * S.nv, hipMemcpyDeviceToHost));
+  return CS_OK;
+}
+
+int cs_write_vtk(cs_solver* h, const char* path) {
+  if (!h || !path) return CS_ERR_ARG;
+  DevState& S = h->S;
+  const int nv = S.nv, ne = S.ne;
+  std::vector<double> x(3 * (size_t)nv), imp(3 * (size_t)nv);
+  std::vector<int32_t> num((size_t)nv);
+  int rc = cs_get_state(h, x.data(), nullptr, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  if ((rc = cs_last_impulses(h, imp.data(), num.data()))) return rc;
+  std::vector<int4> ev((size_t)std::max(ne, 1));
+  HIPCHK(h, hipMemcpy(ev.data(), S.ev, sizeof(int4) * ne, hipMemcpyDeviceToHost));
+  // points numbered in order of first appearance in element order
+  std::vector<int> idx((size_t)nv, -1), order;
+  order.reserve((size_t)nv);
+  for (int e = 0; e < ne; ++e) {
+    const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
+    for (int j = 0; j < 3; ++j)
+      if (vv[j] >= 0 && idx[vv[j]] < 0) {
+        idx[vv[j]] = (int)order.size();
+        order.push_back(vv[j]);
+      }
+  }
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "w"), &std::fclose);
+  if (!f) return fail(h, CS_ERR_ARG, std::string("cannot open ") + path);
+  FILE* o = f.get();
+  std::fprintf(o, "<?xml version=\"1.0\"?>\n<VTKFile type=\"PolyData\" version=\"0.1\" "
+                  "byte_order=\"LittleEndian\">\n<PolyData>\n");
+  std::fprintf(o, "<Piece NumberOfPoints=\"%d\" NumberOfVerts=\"0\" NumberOfLines=\"0\" "
+                  "NumberOfStrips=\"0\" NumberOfPolys=\"%d\">\n", (int)order.size(), ne);
+  std::fprintf(o, "<PointData Vectors=\"CollsnImpulse\">\n<DataArray type=\"Float32\" "
+                  "Name=\"CollsnImpulse\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+  for (int v : order)
+    std::fprintf(o, "%.9g %.9g %.9g\n", (float)imp[3 * v], (float)imp[3 * v + 1], (float)imp[3 * v + 2]);
+  std::fprintf(o, "</DataArray>\n</PointData>\n<CellData Scalars=\"CollsnRegion\">\n"
+                  "<DataArray type=\"UInt8\" Name=\"CollsnRegion\" NumberOfComponents=\"3\" "
+                  "format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) {
+    const int vv[3] = {ev[e].x, ev[e].y, ev[e].z};
+    bool red = false;
+    for (int j = 0; j < 3; ++j)
+      if (vv[j] >= 0 && num[vv[j]] > 0) red = true;
+    std::fprintf(o, red ? "255 0 0\n" : "0 255 0\n");
+  }
+  std::fprintf(o, "</DataArray>\n</CellData>\n<Points>\n<DataArray type=\"Float32\" "
+                  "Name=\"Points\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+  for (int v : order)
+    std::fprintf(o, "%.9g %.9g %.9g\n", (float)x[3 * v], (float)x[3 * v + 1], (float)x[3 * v + 2]);
+  std::fprintf(o, "</DataArray>\n</Points>\n<Polys>\n<DataArray type=\"Int32\" "
+                  "Name=\"connectivity\" format=\"ascii\">\n");
+  for (int e = 0; e < ne; ++e) {
+    if (ev[e].z >= 0) std::fprintf(o, "%d %d %d\n", idx[ev[e].x], idx[ev[e].y], idx[ev[e].z]);
+    else std::fprintf(o, "%d %d\n", idx[ev[e].x], idx[ev[e].y]);
+  }
+  std::fprintf(o, "</DataArray>\n<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n");
+  long long off = 0;
+  for (int e = 0; e < ne; ++e) {
+    off += ev[e].z >= 0 ? 3 : 2;
+    std::fprintf(o, "%lld\n", off);
+  }
+  std::fprintf(o, "</DataArray>\n</Polys>\n</Piece>\n</PolyData>\n</VTKFile>\n");
+  if (std::ferror(o)) return fail(h, CS_ERR_ARG, std::string("write failed: ") + path);
+  return CS_OK;
+}
+
+
+int cs_save(cs_solver* h, const char* path) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!path) return CS_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  CkptHeader hd{};
+  std::memcpy(hd.magic, kCkptMagic, 8);
+  hd.nv = h->S.nv; hd.nt = h->S.nt; hd.nb = h->S.nb; hd.nbody = h->S.nbody;
+  if ((rc = checkpoint_topology(h, &hd.topo))) return rc;
+  hd.params = h->P;
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "wb"), &std::fclose);
+  if (!f) return fail(h, CS_ERR_ARG, std::string("cannot open ") + path);
+  if (std::fwrite(&hd, sizeof(hd), 1, f.get()) != 1) return fail(h, CS_ERR_ARG, "write failed");
+  std::vector<char> buf;
+  for (const CkptArray& a : checkpoint_arrays(h->S)) {
+    buf.resize(a.bytes);
+    HIPCHK(h, hipMemcpy(buf.data(), a.dev, a.bytes, hipMemcpyDeviceToHost));
+    if (a.bytes && std::fwrite(buf.data(), 1, a.bytes, f.get()) != a.bytes)
+      return fail(h, CS_ERR_ARG, "write failed");
+  }
+  return CS_OK;
+}
+
+int cs_load(cs_solver* h, const char* path) {
+  if (!h || !path) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(path, "rb"), &std::fclose);
+  if (!f) return fail(h, CS_ERR_ARG, std::string("cannot open ") + path);
+  CkptHeader hd{};
+  if (std::fread(&hd, sizeof(hd), 1, f.get()) != 1 || std::memcmp(hd.magic, kCkptMagic, 8) != 0)
+    return fail(h, CS_ERR_ARG, "not a checkpoint of this library version");
+  uint64_t topo = 0;
+  int rc = checkpoint_topology(h, &topo);
+  if (rc) return rc;
+  if (hd.nv != h->S.nv || hd.nt != h->S.nt || hd.nb != h->S.nb || hd.nbody != h->S.nbody ||
+      hd.topo != topo)
+    return fail(h, CS_ERR_ARG, "checkpoint topology differs from the assembled mesh");
+  std::vector<char> buf;
+  for (const CkptArray& a : checkpoint_arrays(h->S)) {
+    buf.resize(a.bytes);
+    if (a.bytes && std::fread(buf.data(), 1, a.bytes, f.get()) != a.bytes)
+      return fail(h, CS_ERR_ARG, "checkpoint is truncated");
+    HIPCHK(h, hipMemcpy(a.dev, buf.data(), a.bytes, hipMemcpyHostToDevice));
+  }
+  const int device = h->P.device;
+  h->P = hd.params;
+  h->P.device = device;
+  h->state_set = true;
+  h->prev = PrevPass();
+  h->seed = SeedPairs();
   return CS_OK;
 }
 

@@ -56,7 +56,9 @@ enum {
 };
 
 struct NarrowParams {
-  double dt, thickness, round_eps, k, m, lambda;
+  double dt, thickness, round_eps, k, m, lambda, cr;
+  const int32_t* vbody;  // body of a movable vertex, -1 otherwise (nullptr: no bodies)
+  const double* bmass;   // total_mass of each body
   int ccd;        // 0 proximity, 1 CCD
   int zone_mode;  // merge impact zones (s_detImpZone)
   int log_pairs;
@@ -91,6 +93,24 @@ struct DevState {
   dvec4 *x = nullptr, *x_old = nullptr, *vel = nullptr, *av = nullptr;
   uint8_t *fixed = nullptr, *has = nullptr;
   int32_t* uf = nullptr;  // union-find parent
+  // movable rigid bodies (nbody == 0: none, pointers null).  Every point of a
+  // body holds the same collsnImpulse_RG / collsn_num_RG (an impulse spreads
+  // over the whole body, dcollid.cpp:1101-1115), so they are kept per body.
+  int nbody = 0;
+  int32_t* vbody = nullptr;     // per vertex: body id, -1 if not movable
+  int32_t* bptr = nullptr;      // [nbody + 1] offsets of each body's vertices in bmem
+  uint32_t* bmem = nullptr;     // body vertices, ascending id within a body
+  int32_t* bfirst = nullptr;    // [nbody + 1] offsets in bvisit
+  int32_t* bvisit = nullptr;    // body vertices in order of first appearance in element order
+  double* bmass = nullptr;      // total_mass
+  dvec4* bcom = nullptr;        // center_of_mass
+  dvec4* bvel = nullptr;        // center_of_mass_velo
+  dvec4* bmrg = nullptr;        // the solver's mrg_com copy (collid.h:174)
+  dvec4* rg_imp = nullptr;      // collsnImpulse_RG (never cleared by the solver)
+  int32_t* rg_num = nullptr;    // collsn_num_RG
+  // impulses of the last pass, kept when record_impulses is set
+  dvec4* pimp = nullptr;
+  int32_t* pnum = nullptr;
   // elements
   int4* ev = nullptr;     // (v0, v1, v2 or -1, surface id)
   uint8_t* eflags = nullptr;
@@ -218,9 +238,20 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
                    int npairs, const int* npairs_dev, int cap);
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd);
 void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id);
+// y = f(x) with the step's elementary functions (0 acos, 1 cos, 2 sin, 3 pow(x, 1/3))
+hipError_t launch_math_eval(hipStream_t st, int fn, const double* x, double* y, int64_t n);
 
 // response
-hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount);
+hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount, bool record);
+// movable bodies, after apply_hits: the RG part of updateAverageVelocity
+// (dcollid.cpp:728-734) and, when dt > 0, updateImpactZoneVelocityForRG
+// (dcollid.cpp:267-288): every body zone moved rigidly
+hipError_t apply_bodies(hipStream_t st, DevState& S, double m, double dt);
+// updateImpactZoneVelocity's pass over the body zones (they hold two or more points)
+hipError_t update_body_zones(hipStream_t st, DevState& S, double m, double dt);
+// updateFinalForRG (dcollid.cpp:626-675)
+void launch_final_bodies(hipStream_t st, DevState& S, double dt);
+void launch_record_origin(hipStream_t st, DevState& S);
 void launch_average_velocity(hipStream_t st, DevState& S, double dt);
 void launch_predict(hipStream_t st, DevState& S, double dt);
 void launch_uf_init(hipStream_t st, DevState& S);

@@ -27,19 +27,49 @@ constexpr int NARROW_THREADS = 128;
 struct Sub {
   double xo[4][3];
   double v[4][3];
-  bool st[4];
+  bool st[4];   // static (is_fixed)
+  bool mv[4];   // movable rigid body (is_movableRG)
+  int bd[4];    // body of a movable point
 };
 
+// Per-point impulse and friction increments of one hit.  A contact whose
+// four points are all rigid adds nothing to its points (all_rigid) and
+// spreads imp[0] over body sb[0], imp[1] over body sb[1] instead.
 struct Contrib {
   double imp[4][3];
   double fric[4][3];
+  bool all_rigid;
+  int sb[2];
 };
 
-// PointToTriImpulse for static / deformable points (dcollid3d.cpp:925-1107)
+__device__ inline bool rigid_pt(const Sub& S, int q) { return S.st[q] || S.mv[q]; }
+
+// all-rigid contact: SpreadImpactZoneImpulse(pts[a], rigid_impulse[0]) and
+// (pts[b], -rigid_impulse[1]) for the movable ones (dcollid3d.cpp:1043-1052,
+// 1225-1234)
+__device__ inline void spread(const Sub& S, int a, int b, double r0, double r1, const double nor[3],
+                              Contrib& C) {
+  C.all_rigid = true;
+  C.sb[0] = S.mv[a] ? S.bd[a] : -1;
+  C.sb[1] = S.mv[b] ? S.bd[b] : -1;
+  const double i1 = -1.0 * r1;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    C.imp[0][j] = r0 * nor[j];
+    C.imp[1][j] = i1 * nor[j];
+#pragma unroll
+    for (int q = 2; q < 4; ++q) C.imp[q][j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) C.fric[q][j] = 0.0;
+  }
+}
+
+// PointToTriImpulse (dcollid3d.cpp:925-1107)
 __device__ void d_pt_impulse(const Sub& S, const NarrowParams& P, const double nor[3], double w[3],
                              double dist, Contrib& C, int* err) {
   double v_rel[3] = {0.0, 0.0, 0.0}, vn, vt;
   double impulse = 0.0, m_impulse = 0.0, sum_w = 0.0;
+  double rig[2] = {0.0, 0.0};
   const double k = P.k, m = P.m, dt = P.dt, lambda = P.lambda, h = P.thickness;
   dist = h - dist;
 #pragma unroll
@@ -52,8 +82,23 @@ __device__ void d_pt_impulse(const Sub& S, const NarrowParams& P, const double n
   if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
   else vt = 0.0;
   if (vn < 0) {
-    if (S.st[3] || (S.st[0] && S.st[1] && S.st[2])) impulse = vn;
-    else impulse = vn * 0.5;
+    if (S.st[3] || (S.st[0] && S.st[1] && S.st[2])) {
+      impulse = vn;
+      rig[0] = vn;
+      rig[1] = vn;
+    } else if (S.mv[0] && S.mv[1] && S.mv[2] && S.mv[3]) {
+      const double m1 = P.bmass[S.bd[0]], m2 = P.bmass[S.bd[3]];
+      rig[0] = vn * m2 / (m1 + m2);
+      rig[1] = vn * m1 / (m1 + m2);
+    } else if (S.mv[0] && S.mv[1] && S.mv[2]) {
+      rig[0] = 0.5 * vn;
+      impulse = 0.5 * vn;
+    } else if (S.mv[3]) {
+      impulse = 0.5 * vn;
+      rig[1] = 0.5 * vn;
+    } else {
+      impulse = vn * 0.5;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       if (S.st[i]) w[i] = 0.0;
@@ -65,28 +110,43 @@ __device__ void d_pt_impulse(const Sub& S, const NarrowParams& P, const double n
       for (int i = 0; i < 3; ++i) w[i] = inv * w[i];
     }
   }
-  const bool all_rigid = S.st[0] && S.st[1] && S.st[2] && S.st[3];
+  const bool all_rigid = rigid_pt(S, 0) && rigid_pt(S, 1) && rigid_pt(S, 2) && rigid_pt(S, 3);
   if (vn * dt < 0.1 * dist) {
-    if (!all_rigid) {
+    if (all_rigid) {
+      rig[0] *= 1.0 + P.cr;
+      rig[1] *= 1.0 + P.cr;
+    } else {
       const double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
       impulse += tmp;
+      rig[0] += tmp;
+      rig[1] += tmp;
     }
   }
   if (fabs(sum_w) < CS_MACH_EPS) m_impulse = impulse;
   else m_impulse = 2.0 * impulse / (1.0 + dot3(w, w));
+  if (all_rigid) {
+    spread(S, 0, 3, rig[0], rig[1], nor, C);
+    return;
+  }
+  C.all_rigid = false;
+  C.sb[0] = C.sb[1] = -1;
   const bool fr = fabs(vt) > CS_ROUND_EPS;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
+    const double t_impulse = S.mv[i] ? rig[0] : m_impulse;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      C.imp[i][j] = w[i] * m_impulse * nor[j];
-      C.fric[i][j] = fr ? smax(-fabs(lambda * w[i] * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+      C.imp[i][j] = w[i] * t_impulse * nor[j];
+      C.fric[i][j] = fr ? smax(-fabs(lambda * w[i] * t_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
     }
   }
+  {
+    const double t_impulse = S.mv[3] ? rig[1] : m_impulse;
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    C.imp[3][j] = -(m_impulse * nor[j]);
-    C.fric[3][j] = fr ? smax(-fabs(lambda * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+    for (int j = 0; j < 3; ++j) {
+      C.imp[3][j] = -(t_impulse * nor[j]);
+      C.fric[3][j] = fr ? smax(-fabs(lambda * t_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+    }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -95,10 +155,11 @@ __device__ void d_pt_impulse(const Sub& S, const NarrowParams& P, const double n
       if (!S.st[q] && !isfinite(C.imp[q][j])) atomicOr(err, CS_DEVERR_NONFINITE);
 }
 
-// EdgeToEdgeImpulse for static / deformable points (dcollid3d.cpp:1109-1300)
+// EdgeToEdgeImpulse (dcollid3d.cpp:1109-1300)
 __device__ void d_ee_impulse(const Sub& S, const NarrowParams& P, const double nor[3], double a,
                              double b, double dist, Contrib& C, int* err) {
   double v_rel[3], vn, vt, impulse = 0.0, m_impulse = 0.0;
+  double rig[2] = {0.0, 0.0};
   const double k = P.k, m = P.m, dt = P.dt, lambda = P.lambda, h = P.thickness;
   dist = h - dist;
   double wa0 = 1.0 - a, wa1 = a, wb0 = 1.0 - b, wb1 = b;
@@ -111,31 +172,58 @@ __device__ void d_ee_impulse(const Sub& S, const NarrowParams& P, const double n
   if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
   else vt = 0.0;
   if (vn < 0.0) {
-    if ((S.st[0] && S.st[1]) || (S.st[2] && S.st[3])) impulse = vn;
-    else impulse = vn * 0.5;
+    if ((S.st[0] && S.st[1]) || (S.st[2] && S.st[3])) {
+      impulse = vn;
+      rig[0] = vn;
+      rig[1] = vn;
+    } else if (S.mv[0] && S.mv[1] && S.mv[2] && S.mv[3]) {
+      const double m1 = P.bmass[S.bd[0]], m2 = P.bmass[S.bd[2]];
+      rig[0] = vn * m2 / (m1 + m2);
+      rig[1] = vn * m1 / (m1 + m2);
+    } else if (S.mv[0] && S.mv[1]) {
+      rig[0] = 0.5 * vn;
+      impulse = 0.5 * vn;
+    } else if (S.mv[2] && S.mv[3]) {
+      impulse = 0.5 * vn;
+      rig[1] = 0.5 * vn;
+    } else {
+      impulse = vn * 0.5;
+    }
     if (S.st[0]) wa0 = 0.0;
     if (S.st[1]) wa1 = 0.0;
     if (S.st[2]) wb0 = 0.0;
     if (S.st[3]) wb1 = 0.0;
   }
-  const bool all_rigid = S.st[0] && S.st[1] && S.st[2] && S.st[3];
+  const bool all_rigid = rigid_pt(S, 0) && rigid_pt(S, 1) && rigid_pt(S, 2) && rigid_pt(S, 3);
   if (vn * dt < 0.1 * dist) {
-    if (!all_rigid) {
+    if (all_rigid) {
+      rig[0] *= 1.0 + P.cr;
+      rig[1] *= 1.0 + P.cr;
+    } else {
       const double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
       impulse += tmp;
+      rig[0] += tmp;
+      rig[1] += tmp;
     }
   }
   if (wa0 + wa1 < CS_MACH_EPS || wb0 + wb1 < CS_MACH_EPS) m_impulse = impulse;
   else m_impulse = 2.0 * impulse / (wa0 * wa0 + wa1 * wa1 + wb0 * wb0 + wb1 * wb1);
+  if (all_rigid) {
+    spread(S, 0, 2, rig[0], rig[1], nor, C);
+    return;
+  }
+  C.all_rigid = false;
+  C.sb[0] = C.sb[1] = -1;
   const bool fr = fabs(vt) > CS_ROUND_EPS;
   const double wgt[4] = {wa0, wa1, wb0, wb1};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
+    const double t_impulse = S.mv[q] ? rig[q < 2 ? 0 : 1] : m_impulse;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double d = wgt[q] * m_impulse * nor[j];
+      const double d = wgt[q] * t_impulse * nor[j];
       C.imp[q][j] = (q < 2) ? d : -d;
-      C.fric[q][j] = fr ? smax(-fabs(lambda * wgt[q] * m_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
+      C.fric[q][j] = fr ? smax(-fabs(lambda * wgt[q] * t_impulse / vt), -1.0) * (v_rel[j] - vn * nor[j]) : 0.0;
     }
   }
 #pragma unroll
@@ -294,7 +382,7 @@ __device__ bool d_is_coplanar(const Sub& S, double dt, double roots[4]) {
       roots[2] = -2 * Qsqrt * cs_det_cos((theta - 2 * CS_PI) / 3) - a / 3;
     } else {
       const double sgn = (R > 0) ? 1.0 : -1.0;
-      const double A = -sgn * cs_det_cbrt(fabs(R) + sqrt(R2 - Q3));
+      const double A = -sgn * cs_det_pow_third(fabs(R) + sqrt(R2 - Q3));
       const double B = (fabs(A) < CS_ROUND_EPS) ? 0.0 : Q / A;
       roots[0] = (A + B) - a / 3.0;
       if (fabs(A - B) < CS_ROUND_EPS) roots[1] = roots[2] = -0.5 * (A + B) - a / 3.0;
@@ -421,6 +509,21 @@ __device__ inline void zone_union(const ZoneList& Z, int a, int b) {
   zone_touch(Z, b);
   uf_union(Z.uf, a, b);
 }
+// createImpZone(pts, 4, NO) (dcollid.cpp:473-484): every pair of the four
+// points not holding a movable point is merged, which joins all non-movable
+// points of the sub-test; they are merged through the first of them
+__device__ inline void zone_merge4(const ZoneList& Z, const int32_t* vbody, const int p[4]) {
+  bool ok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ok[i] = !(vbody && vbody[p[i]] >= 0);
+  int hub = -1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (!ok[i]) continue;
+    if (hub < 0) hub = p[i];
+    else zone_union(Z, hub, p[i]);
+  }
+}
 
 // One group of 16 lanes per pair, lane s runs sub-test s (15 for two
 // triangles, 5 for a triangle and a bond, 1 for two bonds), so the dependent
@@ -479,6 +582,8 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
       S.xo[p][0] = xo.x; S.xo[p][1] = xo.y; S.xo[p][2] = xo.z;
       S.v[p][0] = vv.x; S.v[p][1] = vv.y; S.v[p][2] = vv.z;
       S.st[p] = fixed[vid] != 0;
+      S.bd[p] = P.vbody ? P.vbody[vid] : -1;
+      S.mv[p] = S.bd[p] >= 0;
       q[p] = vid;  // from here on: global vertex ids
     }
     hit = d_subtest(S, is_pt, P, C, err);
@@ -494,9 +599,11 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
     if (slot < hit_cap) {
       hit_rec& h = hits[slot];
       h.key = hit_key((uint32_t)B, (uint32_t)A, (uint32_t)s);
+      h.sb[0] = C.sb[0];
+      h.sb[1] = C.sb[1];
 #pragma unroll
       for (int p = 0; p < 4; ++p) {
-        h.v[p] = S.st[p] ? -1 : q[p];
+        h.v[p] = (S.st[p] || C.all_rigid) ? -1 : q[p];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           h.imp[p][k] = C.imp[p][k];
@@ -514,12 +621,7 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
     if (slot < hp_cap) hp[slot] = make_int4(B, A, mask, first_hit);
     atomicAdd(&counters[CNT_HIT_SUB], __popc(mask));
   }
-  if (P.zone_mode && active && s >= first_hit) {
-    // createImpZone(pts, 4): merge every pair of the four points
-    zone_union(Z, q[0], q[1]);
-    zone_union(Z, q[0], q[2]);
-    zone_union(Z, q[0], q[3]);
-  }
+  if (P.zone_mode && active && s >= first_hit) zone_merge4(Z, P.vbody, q);
 }
 
 // npairs_dev != nullptr: the pair count is read on the device (clamped to
@@ -559,7 +661,8 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
 // impact-zone merges of hit pairs carried from the previous pass: the points
 // of every sub-test from the pair's first hit onward
 __global__ void k_hp_unions(const int4* __restrict__ hp, int nmax, const int* __restrict__ n_dev,
-                            const int4* __restrict__ ev, int ccd, ZoneList Z) {
+                            const int4* __restrict__ ev, int ccd, const int32_t* __restrict__ vbody,
+                            ZoneList Z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= min(nmax, *n_dev)) return;
   const int4 r = hp[i];
@@ -582,9 +685,8 @@ __global__ void k_hp_unions(const int4* __restrict__ hp, int nmax, const int* __
     int q[4];
     bool is_pt;
     subtest_points(kind, s, ccd, q, &is_pt);
-    zone_union(Z, g[q[0]], g[q[1]]);
-    zone_union(Z, g[q[0]], g[q[2]]);
-    zone_union(Z, g[q[0]], g[q[3]]);
+    const int p4[4] = {g[q[0]], g[q[1]], g[q[2]], g[q[3]]};
+    zone_merge4(Z, vbody, p4);
   }
 }
 
@@ -593,7 +695,7 @@ __global__ void k_hp_unions(const int4* __restrict__ hp, int nmax, const int* __
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_hp_unions, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n,
-                     S.counters + CNT_HIT_PAIRS, S.ev, ccd, zone_list(S));
+                     S.counters + CNT_HIT_PAIRS, S.ev, ccd, S.vbody, zone_list(S));
 }
 
 // parity log: (pass, lo, hi, mask) for every hit pair of the pass
@@ -610,6 +712,20 @@ void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_hp_log, dim3((n + 255) / 256), dim3(256), 0, st, S.hp, n, pass_id, S.plog,
                      S.plog_cap, S.counters);
+}
+
+__global__ void k_math_eval(int fn, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  y[i] = fn == 0 ? cs_det_acos(v) : fn == 1 ? cs_det_cos(v) : fn == 2 ? cs_det_sin(v)
+                                                                   : cs_det_pow_third(v);
+}
+
+hipError_t launch_math_eval(hipStream_t st, int fn, const double* x, double* y, int64_t n) {
+  hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, fn, x, y, n);
+  CS_HIP_RET(hipGetLastError());
+  return hipStreamSynchronize(st);
 }
 
 }  // namespace cs

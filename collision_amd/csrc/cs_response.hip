@@ -61,9 +61,16 @@ __global__ void k_average_velocity(dvec4* __restrict__ x, const dvec4* __restric
   x[v] = o;
 }
 
-__global__ void k_uf_init(int32_t* __restrict__ uf, int nv) {
+// makeSet (dcollid.cpp:1015-1031) and createImpZoneForRG (dcollid3d.cpp:54-68):
+// every point its own zone, every movable body one zone rooted at its
+// smallest vertex
+__global__ void k_uf_init(int32_t* __restrict__ uf, const int32_t* __restrict__ vbody,
+                          const int32_t* __restrict__ bptr, const uint32_t* __restrict__ bmem,
+                          int nv) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nv) uf[v] = v;
+  if (v >= nv) return;
+  const int b = vbody ? vbody[v] : -1;
+  uf[v] = b >= 0 ? (int32_t)bmem[bptr[b]] : v;
 }
 
 // Sort key of a hit: (lo, hi, sub-test) packed as tightly as the element
@@ -79,34 +86,59 @@ __global__ void k_hit_keys(const hit_rec* __restrict__ hits, int n, int hb,
   idx[i] = (uint32_t)i;
 }
 
-// one record per (hit in canonical order, point); static points get the
-// sentinel key nv and sort last
+// one record per (hit in canonical order, point): key = the vertex; the
+// body spreads of an all-rigid contact use the slots of points 0 and 1 with
+// key nv + body; points without a contribution get the sentinel nv + nbody
+// and sort last
 __global__ void k_expand_records(const hit_rec* __restrict__ hits,
-                                 const uint32_t* __restrict__ sorted_idx, int n, int nv,
+                                 const uint32_t* __restrict__ sorted_idx, int n, int nv, int nbody,
                                  uint32_t* __restrict__ rkey, uint32_t* __restrict__ rval) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 4 * n) return;
   const int r = i >> 2, p = i & 3;
   const uint32_t h = sorted_idx[r];
   const int v = hits[h].v[p];
-  rkey[i] = (v >= 0) ? (uint32_t)v : (uint32_t)nv;
+  const int b = p < 2 ? hits[h].sb[p] : -1;
+  rkey[i] = (v >= 0) ? (uint32_t)v : (b >= 0 ? (uint32_t)(nv + b) : (uint32_t)(nv + nbody));
   rval[i] = h * 4u + (uint32_t)p;
 }
 
 // One thread per vertex segment of the sorted records.  The vertex joins the
 // list of vertices whose average velocity this pass changed (vcount counts
-// it; each vertex heads one segment).
+// it; each vertex heads one segment).  A body segment (key nv + body) adds
+// its spreads to the body's collsnImpulse_RG in canonical order, starting
+// from the value it holds (the reference never clears it), and counts them.
 __global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __restrict__ rval,
-                        int nrec, int nv, const hit_rec* __restrict__ hits,
+                        int nrec, int nv, int nbody, const hit_rec* __restrict__ hits,
                         dvec4* __restrict__ av, uint8_t* __restrict__ has,
+                        dvec4* __restrict__ rg_imp, int32_t* __restrict__ rg_num,
                         int32_t* __restrict__ vlist, int* __restrict__ vcount,
+                        dvec4* __restrict__ pimp, int32_t* __restrict__ pnum,
                         int* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t v = 0;
-  bool head = false;
+  bool head = false, bhead = false;
   if (i < nrec) {
     v = rkey[i];
-    head = v < (uint32_t)nv && !(i > 0 && rkey[i - 1] == v);  // head of the vertex segment
+    const bool first = !(i > 0 && rkey[i - 1] == v);
+    head = first && v < (uint32_t)nv;  // head of the vertex segment
+    bhead = first && v >= (uint32_t)nv && v < (uint32_t)(nv + nbody);
+  }
+  if (bhead) {
+    const int b = (int)v - nv;
+    dvec4 acc = rg_imp[b];
+    int count = 0;
+    for (int j = i; j < nrec && rkey[j] == v; ++j) {
+      const uint32_t hv = rval[j];
+      const hit_rec& h = hits[hv >> 2];
+      const int p = hv & 3;
+      acc.x = acc.x + h.imp[p][0];
+      acc.y = acc.y + h.imp[p][1];
+      acc.z = acc.z + h.imp[p][2];
+      ++count;
+    }
+    rg_imp[b] = acc;
+    rg_num[b] += count;
   }
   if (head) {
     double imp[3] = {0.0, 0.0, 0.0}, fric[3] = {0.0, 0.0, 0.0};
@@ -121,6 +153,10 @@ __global__ void k_apply(const uint32_t* __restrict__ rkey, const uint32_t* __res
         fric[k] += h.fric[p][k];
       }
       ++count;
+    }
+    if (pimp) {
+      pimp[v] = dvec4{imp[0], imp[1], imp[2], 0.0};
+      pnum[v] = count;
     }
     dvec4 a = av[v];
     a.x += (imp[0] + fric[0]) / count;
@@ -141,8 +177,12 @@ static int bits_for(uint64_t maxval) {  // bits needed for values <= maxval, rou
   return b < 8 ? 8 : b;
 }
 
-hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount) {
+hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount, bool record) {
   if (nhits <= 0) return hipSuccess;
+  if (record) {  // a pass with hits replaces the recorded impulses
+    CS_HIP_RET(hipMemsetAsync(S.pimp, 0, sizeof(dvec4) * (size_t)S.nv, st));
+    CS_HIP_RET(hipMemsetAsync(S.pnum, 0, sizeof(int32_t) * (size_t)S.nv, st));
+  }
   const int nb = (nhits + 255) / 256;
   int hb = 1;  // bits of an element id
   while (hb < 32 && ((uint64_t)S.ne >> hb) != 0) ++hb;
@@ -154,13 +194,14 @@ hipError_t apply_hits(hipStream_t st, DevState& S, int nhits, int* vcount) {
   const uint32_t* sidx = in_tmp ? S.hidx_tmp : S.hidx;
   const int nrec = 4 * nhits;
   hipLaunchKernelGGL(k_expand_records, dim3((nrec + 255) / 256), dim3(256), 0, st, S.hits, sidx,
-                     nhits, S.nv, S.rkey, S.rval);
+                     nhits, S.nv, S.nbody, S.rkey, S.rval);
   CS_HIP_RET(radix_sort_u32(st, S.rkey, S.rval, S.rkey_tmp, S.rval_tmp, S.sort_hist, S.scan_part, nrec,
-                            bits_for((uint64_t)S.nv), &in_tmp));
+                            bits_for((uint64_t)S.nv + (uint64_t)S.nbody), &in_tmp));
   const uint32_t* rk = in_tmp ? S.rkey_tmp : S.rkey;
   const uint32_t* rv = in_tmp ? S.rval_tmp : S.rval;
   hipLaunchKernelGGL(k_apply, dim3((nrec + 255) / 256), dim3(256), 0, st, rk, rv, nrec, S.nv,
-                     S.hits, S.av, S.has, S.vlist, vcount, S.counters + CNT_ERR);
+                     S.nbody, S.hits, S.av, S.has, S.rg_imp, S.rg_num, S.vlist, vcount,
+                     record ? S.pimp : nullptr, S.pnum, S.counters + CNT_ERR);
   return hipGetLastError();
 }
 
@@ -234,9 +275,13 @@ struct ZoneMember {
   bool fixed;
 };
 
+// Zones from the step's member list (zkey / zhead / zend, count in
+// counters[CNT_ZONES]) or, with bptr, the movable bodies (members
+// zval[bptr[z] .. bptr[z+1]), nbz zones).
 __global__ void __launch_bounds__(ZONE_LANES)
     k_zone_update(const uint32_t* __restrict__ zkey, const uint32_t* __restrict__ zval,
                   const int32_t* __restrict__ zhead, const int32_t* __restrict__ zend,
+                  const int32_t* __restrict__ bptr, int nbz,
                   const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
                   const uint8_t* __restrict__ fixed, double m, double dt,
                   int* __restrict__ counters) {
@@ -245,10 +290,10 @@ __global__ void __launch_bounds__(ZONE_LANES)
   __shared__ int cid[ZONE_CACHE];  // vertex id, -1 - id when fixed
   int* err = counters + CNT_ERR;
   const int l = threadIdx.x;
-  const int nz = counters[CNT_ZONES];
+  const int nz = bptr ? nbz : counters[CNT_ZONES];
   for (int z = blockIdx.x; z < nz; z += gridDim.x) {
-    const int b = zhead[z];
-    const int e = zend[zkey[b]];
+    const int b = bptr ? bptr[z] : zhead[z];
+    const int e = bptr ? bptr[z + 1] : zend[zkey[b]];
     const int n = e - b;
     // member j (b <= j < e), from the cache when it holds it
     auto member = [&](int j) -> ZoneMember {
@@ -401,8 +446,95 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nl
   hipLaunchKernelGGL(k_zone_heads, dim3(nb), dim3(256), 0, st, zk, nlist, vbits, S.zkey, S.zhead,
                      S.zend, S.counters);
   hipLaunchKernelGGL(k_zone_update, dim3(std::min(nlist, 2048)), dim3(ZONE_LANES), 0, st, S.zkey,
-                     zv, S.zhead, S.zend, S.x_old, S.av, S.fixed, m, dt, S.counters);
+                     zv, S.zhead, S.zend, (const int32_t*)nullptr, 0, S.x_old, S.av, S.fixed, m, dt,
+                     S.counters);
   return hipGetLastError();
+}
+
+hipError_t update_body_zones(hipStream_t st, DevState& S, double m, double dt) {
+  if (S.nbody <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zone_update, dim3(std::min(S.nbody, 2048)), dim3(ZONE_LANES), 0, st,
+                     (const uint32_t*)nullptr, S.bmem, (const int32_t*)nullptr,
+                     (const int32_t*)nullptr, S.bptr, S.nbody, S.x_old, S.av, S.fixed, m, dt,
+                     S.counters);
+  return hipGetLastError();
+}
+
+// The RG part of updateAverageVelocity (dcollid.cpp:728-734), one work-group
+// per body: every point of a body with spreads this pass adds
+// collsnImpulse_RG / collsn_num_RG (after its own impulses, applied by
+// k_apply) and becomes has_collsn; the count is then cleared.
+__global__ void k_body_rg_apply(const int32_t* __restrict__ bptr, const uint32_t* __restrict__ bmem,
+                                dvec4* __restrict__ av, uint8_t* __restrict__ has,
+                                const dvec4* __restrict__ rg_imp, int32_t* __restrict__ rg_num) {
+  const int b = blockIdx.x;
+  const int n = rg_num[b];
+  if (n <= 0) return;  // uniform in the work-group
+  const dvec4 g = rg_imp[b];
+  for (int j = bptr[b] + threadIdx.x; j < bptr[b + 1]; j += blockDim.x) {
+    const uint32_t v = bmem[j];
+    dvec4 a = av[v];
+    a.x += g.x / n;
+    a.y += g.y / n;
+    a.z += g.z / n;
+    av[v] = a;
+    has[v] = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) rg_num[b] = 0;
+}
+
+hipError_t apply_bodies(hipStream_t st, DevState& S, double m, double dt) {
+  if (S.nbody <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_body_rg_apply, dim3(S.nbody), dim3(256), 0, st, S.bptr, S.bmem, S.av, S.has,
+                     S.rg_imp, S.rg_num);
+  if (dt > 0.0) return update_body_zones(st, S, m, dt);
+  return hipGetLastError();
+}
+
+// updateFinalForRG (dcollid.cpp:626-675), one thread per body: the first
+// point of the body (in element order) with has_collsn sets
+// center_of_mass_velo = its avgVel and center_of_mass = avgVel dt + base,
+// where base is the stored mrg_com when that point is the body's very first
+// point and the current center_of_mass otherwise (an earlier point copied it
+// into mrg_com); mrg_com then follows center_of_mass.
+__global__ void k_final_bodies(const int32_t* __restrict__ bfirst, const int32_t* __restrict__ bvisit,
+                               const dvec4* __restrict__ av, const uint8_t* __restrict__ has,
+                               dvec4* __restrict__ bcom, dvec4* __restrict__ bvel,
+                               dvec4* __restrict__ bmrg, double dt, int nbody) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbody) return;
+  for (int j = bfirst[b]; j < bfirst[b + 1]; ++j) {
+    const int v = bvisit[j];
+    if (!has[v]) continue;
+    const dvec4 base = (j == bfirst[b]) ? bmrg[b] : bcom[b];
+    const dvec4 a = av[v];
+    bvel[b] = dvec4{a.x, a.y, a.z, 0.0};
+    bcom[b] = dvec4{a.x * dt + base.x, a.y * dt + base.y, a.z * dt + base.z, 0.0};
+    break;
+  }
+  bmrg[b] = bcom[b];
+}
+
+void launch_final_bodies(hipStream_t st, DevState& S, double dt) {
+  if (S.nbody <= 0) return;
+  hipLaunchKernelGGL(k_final_bodies, dim3((S.nbody + 63) / 64), dim3(64), 0, st, S.bfirst, S.bvisit,
+                     S.av, S.has, S.bcom, S.bvel, S.bmrg, dt, S.nbody);
+}
+
+// recordOriginPosition (dcollid.cpp:91-107)
+__global__ void k_record_origin(const dvec4* __restrict__ x, dvec4* __restrict__ x_old,
+                                uint8_t* __restrict__ has, const int32_t* __restrict__ vbody, int nv) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  has[v] = 0;
+  if (vbody && vbody[v] >= 0) return;
+  x_old[v] = x[v];
+}
+
+void launch_record_origin(hipStream_t st, DevState& S) {
+  hipLaunchKernelGGL(k_record_origin, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old,
+                     S.has, S.vbody, S.nv);
 }
 
 __global__ void k_zone_labels(const int32_t* __restrict__ uf, int nv, int32_t* __restrict__ out) {
@@ -415,39 +547,54 @@ void launch_zone_labels(hipStream_t st, DevState& S, int32_t* out) {
 }
 
 // ---------------------------------------------------------------------------
-// detectDomainBoundaryCollision (dcollid.cpp:116-158); a point's second visit
-// is a no-op, so each point is handled once
+// detectDomainBoundaryCollision (dcollid.cpp:116-158): the reference visits a
+// point once per element holding it (vadj_ptr gives that count).  A visit
+// that leaves the average velocity unchanged leaves every later visit with
+// the same input, so the loop stops there; a later visit can still change a
+// point whose x_old lies outside the box, when the friction of an earlier
+// visit shortens its inward move.  Movable rigid-body points are skipped.
 __global__ void k_domain(dvec4* __restrict__ av, dvec4* __restrict__ x,
-                         const dvec4* __restrict__ x_old, uint8_t* __restrict__ has, int nv,
-                         double lo0, double lo1, double lo2, double hi0, double hi1, double hi2,
-                         double dt, double mu, double thickness) {
+                         const dvec4* __restrict__ x_old, uint8_t* __restrict__ has,
+                         const int32_t* __restrict__ vadj_ptr, const int32_t* __restrict__ vbody,
+                         int nv, double lo0, double lo1, double lo2, double hi0, double hi1,
+                         double hi2, double dt, double mu, double thickness, double mach_eps) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
+  if (vbody && vbody[v] >= 0) return;
+  const int visits = vadj_ptr[v + 1] - vadj_ptr[v];
   dvec4 a4 = av[v], o4 = x_old[v], c4 = x[v];
   double a[3] = {a4.x, a4.y, a4.z}, o[3] = {o4.x, o4.y, o4.z}, c[3] = {c4.x, c4.y, c4.z};
   const double L[3] = {lo0, lo1, lo2}, U[3] = {hi0, hi1, hi2};
-  double dv = 0;
   bool touched = false;
+  for (int visit = 0; visit < visits; ++visit) {
+    const double a0 = a[0], a1 = a[1], a2 = a[2];
+    double dv = 0;
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const double cand = o[j] + dt * a[j];
-    if (cand <= L[j]) {
-      touched = true;
-      dv = fabs(a[j]);
-      a[j] = 0.0;
-      c[j] = L[j] + thickness;
-    } else if (cand >= U[j]) {
-      touched = true;
-      dv = fabs(a[j]);
-      a[j] = 0.0;
-      c[j] = U[j] - thickness;
+    for (int j = 0; j < 3; ++j) {
+      const double cand = o[j] + dt * a[j];
+      if (cand <= L[j]) {
+        touched = true;
+        dv = fabs(a[j]);
+        a[j] = 0.0;
+        c[j] = L[j] + thickness;
+      } else if (cand >= U[j]) {
+        touched = true;
+        dv = fabs(a[j]);
+        a[j] = 0.0;
+        c[j] = U[j] - thickness;
+      }
     }
-  }
-  const double preVt = mag3(a);
-  if (preVt > CS_MACH_EPS) {
-    const double f = smax(1.0 - mu * dv / preVt, 0.0);
+    const double preVt = mag3(a);
+    if (preVt > mach_eps) {
+      const double f = smax(1.0 - mu * dv / preVt, 0.0);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) a[j] *= f;
+      for (int j = 0; j < 3; ++j) a[j] *= f;
+    }
+    // bit patterns: -0.0 and 0.0 differ as results, so compare bits
+    if (__double_as_longlong(a[0]) == __double_as_longlong(a0) &&
+        __double_as_longlong(a[1]) == __double_as_longlong(a1) &&
+        __double_as_longlong(a[2]) == __double_as_longlong(a2))
+      break;
   }
   if (touched) has[v] = 1;
   av[v] = dvec4{a[0], a[1], a[2], 0.0};
@@ -457,7 +604,8 @@ __global__ void k_domain(dvec4* __restrict__ av, dvec4* __restrict__ x,
 void launch_domain_boundary(hipStream_t st, DevState& S, const double lo[3], const double hi[3],
                             double dt, double mu, double thickness) {
   hipLaunchKernelGGL(k_domain, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.av, S.x, S.x_old,
-                     S.has, S.nv, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], dt, mu, thickness);
+                     S.has, S.vadj_ptr, S.vbody, S.nv, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], dt,
+                     mu, thickness, CS_MACH_EPS);
 }
 
 // updateFinalPosition (dcollid.cpp:562-584)
@@ -501,7 +649,8 @@ void launch_average_velocity(hipStream_t st, DevState& S, double dt) {
 }
 
 void launch_uf_init(hipStream_t st, DevState& S) {
-  hipLaunchKernelGGL(k_uf_init, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.uf, S.nv);
+  hipLaunchKernelGGL(k_uf_init, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.uf, S.vbody, S.bptr,
+                     S.bmem, S.nv);
 }
 
 // ---------------------------------------------------------------------------

@@ -149,7 +149,16 @@ class Scene:
     domain_lo: tuple = (-10.0, -10.0, -10.0)
     domain_hi: tuple = (10.0, 10.0, 10.0)
     scene_of_vertex: Optional[np.ndarray] = None   # batched scenes only
+    # movable rigid bodies (MOVABLE_BODY_BOUNDARY surfaces): body of every
+    # vertex (-1: none), total mass and centre of mass of every body
+    vert_body: Optional[np.ndarray] = None
+    body_mass: Optional[np.ndarray] = None
+    body_com: Optional[np.ndarray] = None
     meta: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def nbody(self):
+        return 0 if self.body_mass is None else len(self.body_mass)
 
     @property
     def nv(self):
@@ -172,14 +181,26 @@ class _Builder:
         self.tris: List[np.ndarray] = []
         self.surf: List[np.ndarray] = []
         self.bonds: List[np.ndarray] = []
+        self.body: List[np.ndarray] = []
+        self.bmass: List[float] = []
+        self.bcom: List[np.ndarray] = []
         self.nv = 0
         self.nsurf = 0
 
-    def add_surface(self, x, v, tris, fixed):
+    def add_surface(self, x, v, tris, fixed, movable_mass=None):
+        """A surface; static when `fixed`, a movable rigid body of total mass
+        `movable_mass` (centre of mass = mean of its vertices) when given."""
         base = self.nv
         self.xs.append(np.asarray(x, np.float64))
         self.vs.append(np.asarray(v, np.float64))
         self.fx.append(np.full(len(x), 1 if fixed else 0, np.uint8))
+        if movable_mass is not None:
+            assert not fixed
+            self.body.append(np.full(len(x), len(self.bmass), np.int32))
+            self.bmass.append(float(movable_mass))
+            self.bcom.append(np.asarray(x, np.float64).mean(axis=0))
+        else:
+            self.body.append(np.full(len(x), -1, np.int32))
         self.tris.append(np.asarray(tris, np.int32) + base)
         self.surf.append(np.full(len(tris), self.nsurf, np.int32))
         self.nsurf += 1
@@ -192,6 +213,7 @@ class _Builder:
         self.xs.append(np.asarray(x, np.float64))
         self.vs.append(np.asarray(v, np.float64))
         self.fx.append(np.zeros(len(x), np.uint8))
+        self.body.append(np.full(len(x), -1, np.int32))
         k = np.arange(m, dtype=np.int32)
         self.bonds.append(np.stack([k + base, k + 1 + base], axis=1))
         self.nv += len(x)
@@ -204,11 +226,16 @@ class _Builder:
         tris = np.concatenate(self.tris) if self.tris else np.zeros((0, 3), np.int32)
         surf = np.concatenate(self.surf) if self.surf else np.zeros(0, np.int32)
         bonds = np.concatenate(self.bonds) if self.bonds else np.zeros((0, 2), np.int32)
+        bodies = {}
+        if self.bmass:
+            bodies = dict(vert_body=np.ascontiguousarray(np.concatenate(self.body), np.int32),
+                          body_mass=np.array(self.bmass, np.float64),
+                          body_com=np.ascontiguousarray(np.array(self.bcom, np.float64)))
         return Scene(name=name, x=np.ascontiguousarray(x), vel=np.ascontiguousarray(v),
                      fixed=fixed, tris=np.ascontiguousarray(tris, dtype=np.int32),
                      tri_surf=surf, tri_len0=tri_rest_lengths(x, tris),
                      bonds=np.ascontiguousarray(bonds, dtype=np.int32),
-                     bond_len0=bond_rest_lengths(x, bonds), **kw)
+                     bond_len0=bond_rest_lengths(x, bonds), **bodies, **kw)
 
 
 def edge_length(a: np.ndarray, b: np.ndarray) -> np.ndarray:
@@ -506,6 +533,82 @@ def deck_string_on_static_box(dt=1e-3, steps=150, segs=40):
     b.add_string(p, v)
     return b.build("deck_string_slide_static", thickness=1e-3, dt=dt, steps=steps,
                    friction=0.3)
+
+
+def deck_ball_plane(level=2, dt=1e-3, steps=120):
+    """A movable rigid ball rising into a falling deformable slab
+    (in-ball_plane: sphere at (0.25, 0.25, 0.4), r 0.05, velocity (0, 0, 0.2),
+    MOVABLE_BODY_BOUNDARY; cuboid at z 0.47, 0.15 x 0.15 x 0.01, velocity
+    (0, 0, -0.2)); domain [0, 0.5] x [0, 0.5] x [0.25, 0.75]."""
+    b = _Builder()
+    x, t = icosphere(level, 0.05, center=(0.25, 0.25, 0.4))
+    v = np.zeros_like(x)
+    v[:, 2] = 0.2
+    b.add_surface(x, v, t, fixed=False, movable_mass=0.5)
+    bx, bt = box_mesh((0.25, 0.25, 0.47), (0.15, 0.15, 0.01))
+    bv = np.zeros_like(bx)
+    bv[:, 2] = -0.2
+    b.add_surface(bx, bv, bt, fixed=False)
+    return b.build("deck_ball_plane", thickness=1e-3, dt=dt, steps=steps,
+                   domain_lo=(0.0, 0.0, 0.25), domain_hi=(0.5, 0.5, 0.75))
+
+
+def deck_box_boundary(dt=1e-3, steps=150):
+    """A movable rigid box (in-box_boundary: centre (0.25, 0.25, 0.21), edges
+    0.1 x 0.1 x 0.2, velocity (0.4, 0, -0.4)) falling onto a static plate
+    (contacts between rigid points only: impulses spread over the body) and
+    sliding into a small deformable cloth lying on the plate (mixed
+    contacts); domain [0, 1] x [0, 0.5] x [0, 0.5]."""
+    b = _Builder()
+    x, t = box_mesh((0.25, 0.25, 0.21), (0.1, 0.1, 0.2))
+    v = np.tile([0.4, 0.0, -0.4], (len(x), 1))
+    b.add_surface(x, v, t, fixed=False, movable_mass=2.0)
+    px, pt = box_mesh((0.3, 0.25, 0.08), (0.4, 0.3, 0.04))
+    b.add_surface(px, np.zeros_like(px), pt, fixed=True)
+    cx, ct = cloth_grid(6, 0.06, origin=(0.33, 0.22), z=0.1025)
+    cv = np.zeros_like(cx)
+    cv[:, 2] = -0.05
+    b.add_surface(cx, cv, ct, fixed=False)
+    return b.build("deck_box_boundary", thickness=1e-3, dt=dt, steps=steps, friction=0.2,
+                   domain_lo=(0.0, 0.0, 0.0), domain_hi=(1.0, 0.5, 0.5))
+
+
+def deck_string_slide(dt=1e-3, steps=150, segs=40):
+    """A string sliding down onto a movable cuboid (in-string_slide: curve
+    from (0.25, 0.1, 0.3) to (0.25, 0.25, 0.155) with velocity (0, 0.1, -0.1);
+    cuboid at (0.25, 0.25, 0.13), 0.2 x 0.2 x 0.02, at rest,
+    MOVABLE_BODY_BOUNDARY); the string starts lower so contact comes within
+    the steps run."""
+    b = _Builder()
+    x, t = box_mesh((0.25, 0.25, 0.13), (0.2, 0.2, 0.02))
+    b.add_surface(x, np.zeros_like(x), t, fixed=False, movable_mass=1.0)
+    p = polyline((0.25, 0.1, 0.16), (0.25, 0.25, 0.145), segs)
+    v = np.zeros_like(p)
+    v[:, 1] = 0.1
+    v[:, 2] = -0.1
+    b.add_string(p, v)
+    return b.build("deck_string_slide", thickness=1e-3, dt=dt, steps=steps, friction=0.3,
+                   domain_lo=(0.0, 0.0, 0.0), domain_hi=(0.5, 0.5, 0.5))
+
+
+def domain_wall_strings(dt=1e-3, steps=40, segs=12):
+    """Strings driven into a tight domain box, one of them starting outside
+    it on one axis with an inward velocity, so that the domain-box friction of
+    one visit can make the next visit of the same point touch a face."""
+    b = _Builder()
+    p = polyline((0.02, 0.1, 0.1), (0.02, 0.3, 0.1), segs)
+    v = np.tile([-0.9, 0.3, 0.0], (len(p), 1))
+    b.add_string(p, v)
+    # x_old below the x face; the first visit touches the y face only, its
+    # friction shortens the inward x move, and the second visit touches x
+    q = polyline((-0.0001, 0.4999, 0.1), (-0.0001, 0.4999, 0.3), segs)
+    w = np.tile([0.105, 0.5, -2.5], (len(q), 1))
+    b.add_string(q, w)
+    r = polyline((0.1, 0.1, 0.45), (0.3, 0.1, 0.45), segs)
+    u = np.tile([0.1, -0.4, 1.2], (len(r), 1))
+    b.add_string(r, u)
+    return b.build("domain_wall_strings", thickness=1e-3, dt=dt, steps=steps, friction=0.4,
+                   domain_lo=(0.0, 0.0, 0.0), domain_hi=(0.5, 0.5, 0.5))
 
 
 def fan_disc_on_box(nfan=96, rings=3, vsig=2.0, dt=1e-3, steps=8, seed=7):

@@ -106,24 +106,34 @@ class CollisionSolver3d:
         return (self.params.dom_lo if side == 0 else self.params.dom_hi)[dir]
 
     # -- topology and state --
-    def assemble(self, nv, tris, tri_surf, tri_len0, bonds, bond_len0, is_fixed):
+    def assemble(self, nv, tris, tri_surf, tri_len0, bonds, bond_len0, is_fixed,
+                 vert_body=None, body_mass=None, body_com=None):
         arrs = dict(tris=np.ascontiguousarray(tris, np.int32).reshape(-1, 3),
                     tri_surf=np.ascontiguousarray(tri_surf, np.int32),
                     tri_len0=np.ascontiguousarray(tri_len0, np.float64).reshape(-1, 3),
                     bonds=np.ascontiguousarray(bonds, np.int32).reshape(-1, 2),
                     bond_len0=np.ascontiguousarray(bond_len0, np.float64),
                     is_fixed=np.ascontiguousarray(is_fixed, np.uint8))
+        nbody = 0 if body_mass is None else len(body_mass)
+        if nbody:
+            arrs.update(vert_body=np.ascontiguousarray(vert_body, np.int32),
+                        body_mass=np.ascontiguousarray(body_mass, np.float64),
+                        body_com=np.ascontiguousarray(body_com, np.float64).reshape(-1, 3))
         m = MeshView(nv, arrs["tris"].shape[0], arrs["bonds"].shape[0],
                      _p(arrs["tris"], C.c_int32), _p(arrs["tri_surf"], C.c_int32),
                      _p(arrs["tri_len0"], C.c_double), _p(arrs["bonds"], C.c_int32),
-                     _p(arrs["bond_len0"], C.c_double), _p(arrs["is_fixed"], C.c_uint8))
+                     _p(arrs["bond_len0"], C.c_double), _p(arrs["is_fixed"], C.c_uint8),
+                     nbody, _p(arrs.get("vert_body"), C.c_int32),
+                     _p(arrs.get("body_mass"), C.c_double), _p(arrs.get("body_com"), C.c_double))
         self._check(self.L.cs_assemble(self.h, C.byref(m)))
         self.nv = nv
+        self.nbody = nbody
 
     def assembleFromMesh(self, scene):
         """assembleFromInterface over a flat scene (collision_amd.scenes.Scene)."""
         self.assemble(scene.nv, scene.tris, scene.tri_surf, scene.tri_len0, scene.bonds,
-                      scene.bond_len0, scene.fixed)
+                      scene.bond_len0, scene.fixed, getattr(scene, "vert_body", None),
+                      getattr(scene, "body_mass", None), getattr(scene, "body_com", None))
 
     def setState(self, x, vel, has_collsn=None):
         x = np.ascontiguousarray(x, np.float64)
@@ -189,6 +199,43 @@ class CollisionSolver3d:
         out = np.empty(self.nv, np.int32)
         self._check(self.L.cs_zone_labels(self.h, _p(out, C.c_int32)))
         return out
+
+    def lastImpulses(self):
+        """summed collsnImpulse and collsn_num of every vertex in the last pass"""
+        imp, num = np.empty((self.nv, 3)), np.empty(self.nv, np.int32)
+        self._check(self.L.cs_last_impulses(self.h, _p(imp, C.c_double), _p(num, C.c_int32)))
+        return imp, num
+
+    def writeVtk(self, path):
+        """vtkplotVectorSurface (vtk.cpp:14-93) of the current state"""
+        self._check(self.L.cs_write_vtk(self.h, str(path).encode()))
+
+    def save(self, path):
+        self._check(self.L.cs_save(self.h, str(path).encode()))
+
+    def load(self, path):
+        self._check(self.L.cs_load(self.h, str(path).encode()))
+        p = Params()
+        self._check(self.L.cs_get_params(self.h, C.byref(p)))
+        self.params = p
+
+    def getBodies(self):
+        """(center_of_mass, center_of_mass_velo) of the movable bodies"""
+        nb = getattr(self, "nbody", 0)
+        com, vel = np.zeros((max(nb, 1), 3)), np.zeros((max(nb, 1), 3))
+        self._check(self.L.cs_get_bodies(self.h, _p(com, C.c_double), _p(vel, C.c_double)))
+        return com[:nb], vel[:nb]
+
+
+def math_eval(fn: str, x, device: int = 0) -> np.ndarray:
+    """The step's own acos / cos / sin / pow(x, 1/3), evaluated on the device."""
+    codes = {"acos": 0, "cos": 1, "sin": 2, "pow_third": 3}
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    rc = _lib.lib().cs_math_eval(device, codes[fn], _p(x, C.c_double), _p(y, C.c_double), x.size)
+    if rc:
+        raise CollisionError(rc, "cs_math_eval failed")
+    return y
 
 
 def solver_for_scene(scene, **kw) -> CollisionSolver3d:

@@ -65,7 +65,10 @@ typedef struct cs_params {
                                    times (default), 1 broad- and narrow-phase kernel
                                    times only, 0 none; each timing point is a marker
                                    on the stream, which costs GPU time */
-    int32_t reserved;
+    int32_t record_impulses;    /* 1: keep each vertex's summed collsnImpulse and
+                                   collsn_num of the last pass with a hit (cs_last_impulses,
+                                   cs_write_vtk); the reference's CollisionImpulse
+                                   debugging dump (dcollid.cpp:684-694) */
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
@@ -79,6 +82,14 @@ typedef struct cs_mesh_view {
     const int32_t *bonds;     /* [nb][2] (start, end)                        */
     const double *bond_len0;  /* [nb] rest length                            */
     const uint8_t *is_fixed;  /* [nv] static rigid-body point (STATE.is_fixed) */
+    /* movable rigid bodies (STATE.is_movableRG, cdinit.cpp:201-202); leave
+     * nbody = 0 and the pointers NULL for none.  A body is one surface whose
+     * triangles all have vertices of that body; bonds hold no movable vertex
+     * (the reference's curve points are never movable). */
+    int32_t nbody;
+    const int32_t *vert_body; /* [nv] body of a movable vertex, -1 otherwise      */
+    const double *body_mass;  /* [nbody] total_mass (dcollid3d.cpp:968-969)     */
+    const double *body_com;   /* [nbody][3] center_of_mass (dcollid.cpp:641-650) */
 } cs_mesh_view;
 
 typedef struct cs_pass_info {
@@ -143,9 +154,13 @@ int cs_assemble(cs_solver *h, const cs_mesh_view *mesh);
 int cs_set_state(cs_solver *h, const double *x, const double *vel, const uint8_t *has_collsn);
 int cs_get_state(cs_solver *h, double *x, double *vel, double *avg_vel, double *x_old,
                  uint8_t *has_collsn);
-/* CollisionSolver::recordOriginPosition (dcollid.cpp:91-107): x_old <- Coords,
- * has_collsn <- false */
+/* CollisionSolver::recordOriginPosition (dcollid.cpp:91-107): x_old <- Coords
+ * (movable rigid-body points keep x_old), has_collsn <- false */
 int cs_record_origin_position(cs_solver *h);
+/* center_of_mass and center_of_mass_velo of every movable body as the last
+ * resolve left them (updateFinalForRG, dcollid.cpp:626-675); either pointer
+ * may be NULL.  [nbody][3] each. */
+int cs_get_bodies(cs_solver *h, double *com, double *com_vel);
 
 /* ---- step ---- */
 /* the driver's per-step predictor (test.cpp:192-258): x_old <- x,
@@ -170,6 +185,33 @@ int64_t cs_pair_log_count(cs_solver *h);
 int cs_pair_log(cs_solver *h, int32_t *out4, int64_t cap);
 /* zone label per vertex = smallest vertex id of its impact zone */
 int cs_zone_labels(cs_solver *h, int32_t *out);
+
+/* y[i] = f(x[i]) on the device with the collision step's own elementary
+ * functions (fn 0 acos, 1 cos, 2 sin, 3 pow(x, 1.0/3.0); correctly rounded),
+ * the ones isCoplanar and the impact-zone rotation use (dcollid3d.cpp:163-169,
+ * 436-444); host arrays.  A parity tap with no counterpart in the reference. */
+int cs_math_eval(int32_t device, int32_t fn, const double *x, double *y, int64_t n);
+
+/* ---- diagnostics and restart ---- */
+/* the summed collsnImpulse [nv][3] and collsn_num [nv] every vertex received
+ * in the last pass with a hit of the last resolve (needs record_impulses;
+ * static points, points without a hit and resolves without one hold zero) */
+int cs_last_impulses(cs_solver *h, double *imp, int32_t *num);
+/* vtkplotVectorSurface (vtk.cpp:14-93): a VTK XML PolyData file (.vtp, ASCII)
+ * of the current Coords, one polygon per element in element order, point
+ * vectors "CollsnImpulse" and cell colours "CollsnRegion" (red: a point of
+ * the element had a hit in the pass cs_last_impulses reports, green
+ * otherwise); needs record_impulses */
+int cs_write_vtk(cs_solver *h, const char *path);
+/* checkpoint / restart (the FronTier driver's FT_Save / RestartRun,
+ * test.cpp:72-91, 122-123): the per-vertex state (Coords, x_old, vel,
+ * avgVel, has_collsn), the movable-body state (center_of_mass,
+ * center_of_mass_velo, mrg_com, collsnImpulse_RG) and the parameters.
+ * cs_load needs a handle assembled with the same topology (checked) and
+ * restores everything but the device ordinal; stepping on gives bitwise
+ * the run that was never interrupted. */
+int cs_save(cs_solver *h, const char *path);
+int cs_load(cs_solver *h, const char *path);
 
 /* text of the last error on this handle (or of a failed cs_create if h is NULL) */
 const char *cs_last_error(cs_solver *h);

@@ -12,7 +12,7 @@
  *   Cross3d      = (a1b2-a2b1, a2b0-a0b2, a0b1-a1b0)
  *   Mag3d(a)     = sqrt(Dot3d(a,a))
  *   distance_between_positions(p,q,3) = sqrt(sum_i (p_i-q_i)^2), i = 0,1,2
- *   MACH_EPS     = DBL_EPSILON
+ *   MACH_EPS     = DBL_EPSILON (a parameter, orc_params.mach_eps)
  * (parity unpinned at that boundary: no reference test fixes them).
  *
  * Two processing orders:
@@ -34,7 +34,6 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define MACH_EPS DBL_EPSILON         /* FronTier MACH_EPS (assumed) */
 #define ROUND_EPS 1e-10              /* collid.h:17 */
 #define BOX_EPS 1e-6                 /* AABB.cpp:10-33 */
 #define SUPERELAS_TOL 0.10           /* dcollid.cpp:489 */
@@ -103,7 +102,18 @@ struct orc_solver {
     int32_t *surf;
     double *len0;      /* [ne][3] */
     uint8_t *fixed;
+    /* movable rigid bodies: per vertex body id (-1: none); per body mass,
+     * center_of_mass, center_of_mass_velo, the solver's mrg_com copy
+     * (collid.h:174) and the collsnImpulse_RG / collsn_num_RG accumulators,
+     * which every point of a body holds identically (SpreadImpactZoneImpulse
+     * adds the same amount to all of them, dcollid.cpp:1101-1115) */
+    int32_t *vbody;
+    int32_t nbody;
+    double *bmass, *bcom, *bvel, *mrg_com, *rg_imp;
+    int32_t *rg_num;
     double *x, *x_old, *vel, *av, *imp, *fric;
+    double *last_imp;  /* record_impulses: the last pass's collsnImpulse sums */
+    int32_t *last_num;
     int32_t *num;
     uint8_t *has;
     int32_t *uf_root, *uf_next, *uf_tail, *uf_w;
@@ -122,6 +132,8 @@ struct orc_solver {
     int32_t *stack;
 };
 
+static int elem_is_tri(const orc_solver *s, int32_t e) { return s->ev[3 * e + 2] >= 0; }
+
 void orc_default_params(orc_params *p) {
     memset(p, 0, sizeof(*p));
     p->dt = 1e-3;            /* DT, collid.h:19 */
@@ -138,6 +150,7 @@ void orc_default_params(orc_params *p) {
     p->order_mode = ORC_ORDER_CANONICAL;
     p->math_mode = ORC_MATH_DET;
     p->log_pairs = 0;
+    p->mach_eps = DBL_EPSILON;
 }
 
 orc_solver *orc_create(const orc_params *p) {
@@ -162,6 +175,11 @@ static void free_topology(orc_solver *s) {
     free(s->num); free(s->has);
     free(s->uf_root); free(s->uf_next); free(s->uf_tail); free(s->uf_w); free(s->sorted);
     free(s->ebox); free(s->visited); free(s->stack);
+    free(s->vbody); free(s->bmass); free(s->bcom); free(s->bvel); free(s->mrg_com);
+    free(s->rg_imp); free(s->rg_num); free(s->last_imp); free(s->last_num);
+    s->last_imp = NULL; s->last_num = NULL;
+    s->vbody = NULL; s->bmass = s->bcom = s->bvel = s->mrg_com = s->rg_imp = NULL;
+    s->rg_num = NULL; s->nbody = 0;
     tree_free(&s->prox); tree_free(&s->coll);
 }
 
@@ -205,6 +223,8 @@ int orc_assemble(orc_solver *s, int32_t nv, int32_t nt, int32_t nb,
     s->sorted = (uint8_t *)calloc((size_t)nv, 1);
     s->ebox = (box_t *)malloc(sizeof(box_t) * (size_t)(ne ? ne : 1));
     s->visited = (uint8_t *)calloc((size_t)(ne ? ne : 1), 1);
+    s->vbody = (int32_t *)malloc(sizeof(int32_t) * (size_t)nv);
+    for (int32_t v = 0; v < nv; ++v) s->vbody[v] = -1;
     s->stack = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * ne + 64));
     for (int32_t e = 0; e < nt; ++e) {
         for (int j = 0; j < 3; ++j) {
@@ -240,6 +260,11 @@ int orc_set_state(orc_solver *s, const double *x, const double *vel, const uint8
     memset(s->fric, 0, n3 * sizeof(double));
     memset(s->num, 0, (size_t)s->nv * sizeof(int32_t));
     memset(s->av, 0, n3 * sizeof(double));
+    /* initSurfaceState clears the RG accumulators (cdinit.cpp:192-197) */
+    if (s->nbody) {
+        memset(s->rg_imp, 0, 3 * (size_t)s->nbody * sizeof(double));
+        memset(s->rg_num, 0, (size_t)s->nbody * sizeof(int32_t));
+    }
     return ORC_OK;
 }
 
@@ -251,6 +276,58 @@ int orc_get_state(orc_solver *s, double *x, double *vel, double *av, double *x_o
     if (av) memcpy(av, s->av, n3 * sizeof(double));
     if (x_old) memcpy(x_old, s->x_old, n3 * sizeof(double));
     if (has) memcpy(has, s->has, (size_t)s->nv);
+    return ORC_OK;
+}
+
+int orc_set_bodies(orc_solver *s, int32_t nbody, const int32_t *vert_body,
+                   const double *body_mass, const double *body_com) {
+    if (!s || !s->x || nbody < 0 || (nbody > 0 && (!vert_body || !body_mass || !body_com)))
+        return ORC_ERR_ARG;
+    free(s->bmass); free(s->bcom); free(s->bvel); free(s->mrg_com); free(s->rg_imp); free(s->rg_num);
+    s->bmass = s->bcom = s->bvel = s->mrg_com = s->rg_imp = NULL;
+    s->rg_num = NULL;
+    s->nbody = 0;
+    for (int32_t v = 0; v < s->nv; ++v) {
+        int32_t b = vert_body ? vert_body[v] : -1;
+        if (b < -1 || b >= nbody) return ORC_ERR_ARG;
+        if (b >= 0 && s->fixed[v]) return ORC_ERR_ARG;
+        s->vbody[v] = b;
+    }
+    /* a body is a set of whole triangles; bonds hold no movable point */
+    for (int32_t e = 0; e < s->ne; ++e) {
+        int np = elem_is_tri(s, e) ? 3 : 2;
+        int32_t b0 = s->vbody[s->ev[3 * e]];
+        for (int i = 0; i < np; ++i) {
+            int32_t b = s->vbody[s->ev[3 * e + i]];
+            if (b != b0 || (np == 2 && b >= 0)) return ORC_ERR_ARG;
+        }
+    }
+    size_t nb3 = 3 * (size_t)(nbody ? nbody : 1);
+    s->bmass = (double *)calloc((size_t)(nbody ? nbody : 1), sizeof(double));
+    s->bcom = (double *)calloc(nb3, sizeof(double));
+    s->bvel = (double *)calloc(nb3, sizeof(double));
+    s->mrg_com = (double *)calloc(nb3, sizeof(double));
+    s->rg_imp = (double *)calloc(nb3, sizeof(double));
+    s->rg_num = (int32_t *)calloc((size_t)(nbody ? nbody : 1), sizeof(int32_t));
+    if (nbody) {
+        memcpy(s->bmass, body_mass, sizeof(double) * (size_t)nbody);
+        memcpy(s->bcom, body_com, sizeof(double) * 3 * (size_t)nbody);
+        memcpy(s->mrg_com, body_com, sizeof(double) * 3 * (size_t)nbody);
+    }
+    s->nbody = nbody;
+    return ORC_OK;
+}
+
+int orc_get_bodies(orc_solver *s, double *com, double *com_vel) {
+    if (!s) return ORC_ERR_ARG;
+    if (com && s->nbody) memcpy(com, s->bcom, sizeof(double) * 3 * (size_t)s->nbody);
+    if (com_vel && s->nbody) memcpy(com_vel, s->bvel, sizeof(double) * 3 * (size_t)s->nbody);
+    return ORC_OK;
+}
+
+int orc_set_avg_vel(orc_solver *s, const double *avg_vel) {
+    if (!s || !s->x || !avg_vel) return ORC_ERR_ARG;
+    memcpy(s->av, avg_vel, sizeof(double) * 3 * (size_t)s->nv);
     return ORC_OK;
 }
 
@@ -268,25 +345,46 @@ int orc_predict(orc_solver *s) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* elementary math selected by mode */
-static inline double m_acos(const orc_solver *s, double x) {
-    return s->P.math_mode == ORC_MATH_LIBM ? acos(x) : orc_det_acos(x);
+/* elementary math selected by mode.  In libm mode the calls whose libm
+ * result differs from the correctly rounded one are counted (a diagnostic:
+ * results may only diverge between the two modes after such a call). */
+static int64_t g_libm_differs = 0;
+static inline double libm_checked(double v, double cr) {
+    if (v != cr && !(v != v && cr != cr)) g_libm_differs++;
+    return v;
 }
-static inline double m_cos(int mode, double x) { return mode == ORC_MATH_LIBM ? cos(x) : orc_det_cos(x); }
-static inline double m_sin(int mode, double x) { return mode == ORC_MATH_LIBM ? sin(x) : orc_det_sin(x); }
+static inline double m_acos(const orc_solver *s, double x) {
+    return s->P.math_mode == ORC_MATH_LIBM ? libm_checked(acos(x), orc_det_acos(x)) : orc_det_acos(x);
+}
+static inline double m_cos(int mode, double x) {
+    return mode == ORC_MATH_LIBM ? libm_checked(cos(x), orc_det_cos(x)) : orc_det_cos(x);
+}
+static inline double m_sin(int mode, double x) {
+    return mode == ORC_MATH_LIBM ? libm_checked(sin(x), orc_det_sin(x)) : orc_det_sin(x);
+}
 static inline double m_cbrt(int mode, double x) {
-    return mode == ORC_MATH_LIBM ? pow(x, 1.0 / 3.0) : orc_det_cbrt(x);
+    return mode == ORC_MATH_LIBM ? libm_checked(pow(x, 1.0 / 3.0), orc_det_pow_third(x))
+                                 : orc_det_pow_third(x);
+}
+int64_t orc_libm_misrounded(int32_t reset) {
+    int64_t n = g_libm_differs;
+    if (reset) g_libm_differs = 0;
+    return n;
 }
 
 /* ------------------------------------------------------------------------ */
-/* per-vertex flags (dcollid.cpp:1069-1099); movable rigid bodies are not
- * represented: every point is either static (is_fixed) or deformable. */
+/* per-vertex flags (dcollid.cpp:1069-1099): static (is_fixed), movable
+ * (is_movableRG), rigid = either */
 static inline int is_static(const uint8_t *fixed, int32_t v) { return fixed[v] != 0; }
 
 /* impulse context: the accumulator arrays and parameters */
 typedef struct {
     const orc_params *P;
     const uint8_t *fixed;
+    const int32_t *vbody;  /* body of a movable vertex, -1 otherwise */
+    const double *bmass;   /* total_mass per body */
+    double *rg_imp;        /* collsnImpulse_RG per body */
+    int32_t *rg_num;       /* collsn_num_RG per body */
     const double *av;      /* avgVel, indexed by vertex id */
     const double *x_old;
     double *imp, *fric;
@@ -294,14 +392,31 @@ typedef struct {
     int err;
 } imp_ctx;
 
-/* PointToTriImpulse, dcollid3d.cpp:925-1107 (static / deformable points) */
+static inline int c_static(const imp_ctx *C, int32_t v) { return C->fixed[v] != 0; }
+static inline int c_movable(const imp_ctx *C, int32_t v) { return C->vbody && C->vbody[v] >= 0; }
+static inline int c_rigid(const imp_ctx *C, int32_t v) { return c_static(C, v) || c_movable(C, v); }
+
+/* SpreadImpactZoneImpulse (dcollid.cpp:1101-1115): the zone of a movable
+ * point is its body (createImpZone never merges movable points outside
+ * createImpZoneForRG, dcollid.cpp:473-484), so every point of the body
+ * receives impulse * nor and one more count */
+static void spread_rg(imp_ctx *C, int32_t v, double impulse, const double nor[3]) {
+    int32_t b = C->vbody[v];
+    for (int i = 0; i < 3; ++i) C->rg_imp[3 * b + i] += impulse * nor[i];
+    C->rg_num[b] += 1;
+}
+
+/* PointToTriImpulse, dcollid3d.cpp:925-1107 */
 static void pt_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double w[3],
                        double dist, double root) {
     (void)root;
     const orc_params *P = C->P;
+    const double eps_m = P->mach_eps;
     double v_rel[3] = {0.0, 0.0, 0.0}, vn, vt;
     double impulse = 0.0, m_impulse = 0.0, sum_w = 0.0;
+    double rigid_impulse[2] = {0.0, 0.0};
     double k = P->spring_k, m = P->mass, dt = P->dt, lambda = P->friction, h = P->thickness;
+    double cr = P->restitution;
     dist = h - dist;
     for (int i = 0; i < 3; ++i) {
         v_rel[i] += C->av[3 * p[3] + i];
@@ -310,33 +425,58 @@ static void pt_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double w[3
     vn = dot3(v_rel, nor);
     if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
     else vt = 0.0;
-    int s0 = is_static(C->fixed, p[0]), s1 = is_static(C->fixed, p[1]);
-    int s2 = is_static(C->fixed, p[2]), s3 = is_static(C->fixed, p[3]);
+    int s0 = c_static(C, p[0]), s1 = c_static(C, p[1]), s2 = c_static(C, p[2]), s3 = c_static(C, p[3]);
+    int m0 = c_movable(C, p[0]), m1 = c_movable(C, p[1]), m2 = c_movable(C, p[2]), m3 = c_movable(C, p[3]);
     if (vn < 0) {
-        if (s3 || (s0 && s1 && s2)) impulse = vn;
-        else impulse = vn * 0.5;
+        if (s3 || (s0 && s1 && s2)) {
+            impulse = vn;
+            rigid_impulse[0] = vn;
+            rigid_impulse[1] = vn;
+        } else if (m0 && m1 && m2 && m3) {
+            double ma = C->bmass[C->vbody[p[0]]], mb = C->bmass[C->vbody[p[3]]];
+            rigid_impulse[0] = vn * mb / (ma + mb);
+            rigid_impulse[1] = vn * ma / (ma + mb);
+        } else if (m0 && m1 && m2) {
+            rigid_impulse[0] = 0.5 * vn;
+            impulse = 0.5 * vn;
+        } else if (m3) {
+            impulse = 0.5 * vn;
+            rigid_impulse[1] = 0.5 * vn;
+        } else {
+            impulse = vn * 0.5;
+        }
         for (int i = 0; i < 3; ++i) {
-            if (is_static(C->fixed, p[i])) w[i] = 0.0;
+            if (c_static(C, p[i])) w[i] = 0.0;
             sum_w += w[i];
         }
-        if (fabs(sum_w) > MACH_EPS) {
+        if (fabs(sum_w) > eps_m) {
             double inv = 1.0 / sum_w;
             for (int i = 0; i < 3; ++i) w[i] = inv * w[i];
         }
     }
-    int all_rigid = s0 && s1 && s2 && s3;
+    int all_rigid = c_rigid(C, p[0]) && c_rigid(C, p[1]) && c_rigid(C, p[2]) && c_rigid(C, p[3]);
     if (vn * dt < 0.1 * dist) {
-        if (!all_rigid) {
+        if (all_rigid) {
+            rigid_impulse[0] *= 1.0 + cr;
+            rigid_impulse[1] *= 1.0 + cr;
+        } else {
             double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
             impulse += tmp;
+            rigid_impulse[0] += tmp;
+            rigid_impulse[1] += tmp;
         }
     }
-    if (fabs(sum_w) < MACH_EPS) m_impulse = impulse;
+    if (fabs(sum_w) < eps_m) m_impulse = impulse;
     else m_impulse = 2.0 * impulse / (1.0 + dot3(w, w));
-    if (all_rigid) return;  /* only movable bodies would receive anything */
+    if (all_rigid) {
+        if (m0) spread_rg(C, p[0], rigid_impulse[0], nor);
+        if (m3) spread_rg(C, p[3], -1.0 * rigid_impulse[1], nor);
+        return;
+    }
     for (int i = 0; i < 3; ++i) {
-        if (is_static(C->fixed, p[i])) continue;
+        if (c_static(C, p[i])) continue;
         double t_impulse = m_impulse;
+        if (c_movable(C, p[i])) t_impulse = rigid_impulse[0];
         for (int j = 0; j < 3; ++j) {
             C->imp[3 * p[i] + j] += w[i] * t_impulse * nor[j];
             if (fabs(vt) > ROUND_EPS)
@@ -347,6 +487,7 @@ static void pt_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double w[3
     }
     if (!s3) {
         double t_impulse = m_impulse;
+        if (m3) t_impulse = rigid_impulse[1];
         for (int j = 0; j < 3; ++j) {
             C->imp[3 * p[3] + j] -= t_impulse * nor[j];
             if (fabs(vt) > ROUND_EPS)
@@ -363,13 +504,16 @@ static void pt_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double w[3
         }
 }
 
-/* EdgeToEdgeImpulse, dcollid3d.cpp:1109-1300 (static / deformable points) */
+/* EdgeToEdgeImpulse, dcollid3d.cpp:1109-1300 */
 static void ee_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double a, double b,
                        double dist, double root) {
     (void)root;
     const orc_params *P = C->P;
+    const double eps_m = P->mach_eps;
     double v_rel[3], vn, vt, impulse = 0.0, m_impulse = 0.0;
+    double rigid_impulse[2] = {0.0, 0.0};
     double k = P->spring_k, m = P->mass, dt = P->dt, lambda = P->friction, h = P->thickness;
+    double cr = P->restitution;
     dist = h - dist;
     double wa[2] = {1.0 - a, a}, wb[2] = {1.0 - b, b};
     for (int j = 0; j < 3; ++j) {
@@ -379,31 +523,56 @@ static void ee_impulse(imp_ctx *C, const int32_t p[4], double nor[3], double a, 
     vn = dot3(v_rel, nor);
     if (dot3(v_rel, v_rel) > vn * vn) vt = sqrt(dot3(v_rel, v_rel) - vn * vn);
     else vt = 0.0;
-    int s0 = is_static(C->fixed, p[0]), s1 = is_static(C->fixed, p[1]);
-    int s2 = is_static(C->fixed, p[2]), s3 = is_static(C->fixed, p[3]);
+    int s0 = c_static(C, p[0]), s1 = c_static(C, p[1]), s2 = c_static(C, p[2]), s3 = c_static(C, p[3]);
+    int m0 = c_movable(C, p[0]), m1 = c_movable(C, p[1]), m2 = c_movable(C, p[2]), m3 = c_movable(C, p[3]);
     if (vn < 0.0) {
-        if ((s0 && s1) || (s2 && s3)) impulse = vn;
-        else impulse = vn * 0.5;
+        if ((s0 && s1) || (s2 && s3)) {
+            impulse = vn;
+            rigid_impulse[0] = vn;
+            rigid_impulse[1] = vn;
+        } else if (m0 && m1 && m2 && m3) {
+            double ma = C->bmass[C->vbody[p[0]]], mb = C->bmass[C->vbody[p[2]]];
+            rigid_impulse[0] = vn * mb / (ma + mb);
+            rigid_impulse[1] = vn * ma / (ma + mb);
+        } else if (m0 && m1) {
+            rigid_impulse[0] = 0.5 * vn;
+            impulse = 0.5 * vn;
+        } else if (m2 && m3) {
+            impulse = 0.5 * vn;
+            rigid_impulse[1] = 0.5 * vn;
+        } else {
+            impulse = vn * 0.5;
+        }
         if (s0) wa[0] = 0.0;
         if (s1) wa[1] = 0.0;
         if (s2) wb[0] = 0.0;
         if (s3) wb[1] = 0.0;
     }
-    int all_rigid = s0 && s1 && s2 && s3;
+    int all_rigid = c_rigid(C, p[0]) && c_rigid(C, p[1]) && c_rigid(C, p[2]) && c_rigid(C, p[3]);
     if (vn * dt < 0.1 * dist) {
-        if (!all_rigid) {
+        if (all_rigid) {
+            rigid_impulse[0] *= 1.0 + cr;
+            rigid_impulse[1] *= 1.0 + cr;
+        } else {
             double tmp = -smin(dt * k * dist / m, (0.1 * dist / dt - vn));
             impulse += tmp;
+            rigid_impulse[0] += tmp;
+            rigid_impulse[1] += tmp;
         }
     }
-    if (wa[0] + wa[1] < MACH_EPS || wb[0] + wb[1] < MACH_EPS) m_impulse = impulse;
+    if (wa[0] + wa[1] < eps_m || wb[0] + wb[1] < eps_m) m_impulse = impulse;
     else m_impulse = 2.0 * impulse / (wa[0] * wa[0] + wa[1] * wa[1] + wb[0] * wb[0] + wb[1] * wb[1]);
-    if (all_rigid) return;
+    if (all_rigid) {
+        if (m0) spread_rg(C, p[0], rigid_impulse[0], nor);
+        if (m2) spread_rg(C, p[2], -1.0 * rigid_impulse[1], nor);
+        return;
+    }
     const double wgt[4] = {wa[0], wa[1], wb[0], wb[1]};
     for (int j = 0; j < 3; ++j) {
         for (int q = 0; q < 4; ++q) {
-            if (is_static(C->fixed, p[q])) continue;
+            if (c_static(C, p[q])) continue;
             double t_impulse = m_impulse;
+            if (c_movable(C, p[q])) t_impulse = rigid_impulse[q < 2 ? 0 : 1];
             if (q < 2) C->imp[3 * p[q] + j] += wgt[q] * t_impulse * nor[j];
             else C->imp[3 * p[q] + j] -= wgt[q] * t_impulse * nor[j];
             if (fabs(vt) > ROUND_EPS)
@@ -429,7 +598,7 @@ static int point_to_tri(imp_ctx *C, const int32_t p[4], const double c[4][3], do
     sub3(c[1], c[2], x23);
     sub3(c[3], c[2], x43);
     det = dot3(x13, x13) * dot3(x23, x23) - dot3(x13, x23) * dot3(x13, x23);
-    if (fabs(det) < 1000 * MACH_EPS) return 0;   /* degenerate triangle */
+    if (fabs(det) < 1000 * C->P->mach_eps) return 0;   /* degenerate triangle */
     cross3(x13, x23, nor);
     nor_mag = mag3(nor);
     double x43_old[3];
@@ -482,7 +651,7 @@ static int edge_to_edge(imp_ctx *C, const int32_t p[4], const double c[4][3], do
     for (int i = 0; i < 3; ++i) { v1[i] = c[0][i] + v1[i]; v2[i] = c[2][i] + v2[i]; }
     sub3(v2, v1, nor);
     nor_mag = mag3(nor);
-    if (nor_mag < 1000 * MACH_EPS) {
+    if (nor_mag < 1000 * C->P->mach_eps) {
         const double *xo = C->x_old;
         for (int j = 0; j < 3; ++j) {
             nor[j] = (1.0 - b) * xo[3 * p[2] + j] + b * xo[3 * p[3] + j];
@@ -492,7 +661,7 @@ static int edge_to_edge(imp_ctx *C, const int32_t p[4], const double c[4][3], do
     dist = dist3(v1, v2);
     if (dist > h) return 0;
     nor_mag = mag3(nor);
-    if (nor_mag < MACH_EPS) {
+    if (nor_mag < C->P->mach_eps) {
         C->err = ORC_ERR_NONFINITE;              /* "Normal vector is nan" */
         return 0;
     }
@@ -502,8 +671,8 @@ static int edge_to_edge(imp_ctx *C, const int32_t p[4], const double c[4][3], do
 }
 
 /* isCoplanar, dcollid3d.cpp:371-482 */
-static int is_coplanar(int math_mode, const double xo[4][3], const double vv4[4][3],
-                       double dt, double roots[4]) {
+static int is_coplanar(int math_mode, double eps_m, const double xo[4][3],
+                       const double vv4[4][3], double dt, double roots[4]) {
     double v[4][3], x[4][3];
     for (int k = 0; k < 3; ++k) { v[0][k] = vv4[0][k]; x[0][k] = xo[0][k]; }
     for (int i = 1; i < 4; ++i)
@@ -529,7 +698,7 @@ static int is_coplanar(int math_mode, const double xo[4][3], const double vv4[4]
     d = x[3][0] * xx[0] - x[3][1] * xx[1] + x[3][2] * xx[2];
     roots[0] = roots[1] = roots[2] = -1.0;
     roots[3] = dt;
-    if (fabs(a) > MACH_EPS) {
+    if (fabs(a) > eps_m) {
         b /= a; c /= a; d /= a;
         a = b; b = c; c = d;
         double Q = (a * a - 3 * b) / 9;
@@ -538,7 +707,8 @@ static int is_coplanar(int math_mode, const double xo[4][3], const double vv4[4]
         if (R2 < Q3) {
             double Qsqrt = sqrt(Q);
             double theta;
-            theta = (math_mode == ORC_MATH_LIBM) ? acos(R / sqrt(Q3)) : orc_det_acos(R / sqrt(Q3));
+            theta = (math_mode == ORC_MATH_LIBM) ? libm_checked(acos(R / sqrt(Q3)), orc_det_acos(R / sqrt(Q3)))
+                                                  : orc_det_acos(R / sqrt(Q3));
             roots[0] = -2 * Qsqrt * m_cos(math_mode, theta / 3) - a / 3;
             roots[1] = -2 * Qsqrt * m_cos(math_mode, (theta + 2 * M_PI) / 3) - a / 3;
             roots[2] = -2 * Qsqrt * m_cos(math_mode, (theta - 2 * M_PI) / 3) - a / 3;
@@ -562,14 +732,14 @@ static int is_coplanar(int math_mode, const double xo[4][3], const double vv4[4]
         }
     }
     for (int i = 0; i < 3; ++i) {
-        roots[i] = roots[i] - MACH_EPS;
+        roots[i] = roots[i] - eps_m;
         if (roots[i] < 0 || roots[i] > dt) roots[i] = -1;
     }
     double t;
     if (roots[0] > roots[1]) { t = roots[0]; roots[0] = roots[1]; roots[1] = t; }
     if (roots[0] > roots[2]) { t = roots[0]; roots[0] = roots[2]; roots[2] = t; }
     if (roots[1] > roots[2]) { t = roots[1]; roots[1] = roots[2]; roots[2] = t; }
-    return (roots[0] > MACH_EPS || roots[1] > MACH_EPS || roots[2] > MACH_EPS) ? 1 : 0;
+    return (roots[0] > eps_m || roots[1] > eps_m || roots[2] > eps_m) ? 1 : 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -595,10 +765,28 @@ static void uf_merge(orc_solver *s, int32_t X, int32_t Y) {
         s->uf_tail[PY] = s->uf_tail[PX];
     }
 }
-/* createImpZone(pts, 4, NO), dcollid.cpp:473-484 (no movable bodies) */
-static void create_imp_zone(orc_solver *s, const int32_t p[4]) {
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < i; ++j) uf_merge(s, p[i], p[j]);
+/* createImpZone(pts, num, first), dcollid.cpp:473-484: outside the body
+ * zones built at assembly (first), pairs with a movable point are skipped */
+static inline int is_movable(const orc_solver *s, int32_t v) { return s->nbody && s->vbody[v] >= 0; }
+static void create_imp_zone_n(orc_solver *s, const int32_t *p, int num, int first) {
+    for (int i = 0; i < num; ++i)
+        for (int j = 0; j < i; ++j) {
+            if (!first && (is_movable(s, p[i]) || is_movable(s, p[j]))) continue;
+            uf_merge(s, p[i], p[j]);
+        }
+}
+static void create_imp_zone(orc_solver *s, const int32_t p[4]) { create_imp_zone_n(s, p, 4, 0); }
+
+/* makeSet (dcollid.cpp:1015-1031) and createImpZoneForRG
+ * (dcollid3d.cpp:54-68): one zone per movable body, merged triangle by
+ * triangle in element order */
+static void make_set(orc_solver *s) {
+    for (int32_t v = 0; v < s->nv; ++v) {
+        s->uf_root[v] = v; s->uf_next[v] = -1; s->uf_tail[v] = v; s->uf_w[v] = 1;
+    }
+    if (!s->nbody) return;
+    for (int32_t e = 0; e < s->nt; ++e)
+        if (is_movable(s, s->ev[3 * e])) create_imp_zone_n(s, &s->ev[3 * e], 3, 1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -615,7 +803,8 @@ static int subtest(orc_solver *s, imp_ctx *C, int is_pt, const int32_t p[4], int
     double xo[4][3], v4[4][3], roots[4];
     for (int q = 0; q < 4; ++q)
         for (int k = 0; k < 3; ++k) { xo[q][k] = s->x_old[3 * p[q] + k]; v4[q][k] = s->av[3 * p[q] + k]; }
-    if (!is_coplanar(s->P.math_mode, (const double(*)[3])xo, (const double(*)[3])v4, s->P.dt, roots))
+    if (!is_coplanar(s->P.math_mode, s->P.mach_eps, (const double(*)[3])xo, (const double(*)[3])v4,
+                     s->P.dt, roots))
         return 0;
     for (int i = 0; i < 4; ++i) {
         if (roots[i] < 0) continue;
@@ -628,10 +817,13 @@ static int subtest(orc_solver *s, imp_ctx *C, int is_pt, const int32_t p[4], int
     return 0;
 }
 
-static int elem_is_tri(const orc_solver *s, int32_t e) { return s->ev[3 * e + 2] >= 0; }
+/* isRigidBody(const CD_HSE*) (dcollid.cpp:1069-1099): a static or movable point */
 static int elem_is_rigid(const orc_solver *s, int32_t e) {
     int np = elem_is_tri(s, e) ? 3 : 2;
-    for (int i = 0; i < np; ++i) if (s->fixed[s->ev[3 * e + i]]) return 1;
+    for (int i = 0; i < np; ++i) {
+        int32_t v = s->ev[3 * e + i];
+        if (s->fixed[v] || is_movable(s, v)) return 1;
+    }
     return 0;
 }
 
@@ -944,7 +1136,8 @@ static int run_pass(orc_solver *s, int kind) {
         }
         qsort(pp, (size_t)np, sizeof(int64_t), cmp_i64);
     }
-    imp_ctx C = {&s->P, s->fixed, s->av, s->x_old, s->imp, s->fric, s->num, 0};
+    imp_ctx C = {&s->P, s->fixed, s->nbody ? s->vbody : NULL, s->bmass, s->rg_imp, s->rg_num,
+                 s->av, s->x_old, s->imp, s->fric, s->num, 0};
     orc_pass_info *pi = &s->cur;
     memset(pi, 0, sizeof(*pi));
     pi->kind = kind;
@@ -977,7 +1170,27 @@ static int run_pass(orc_solver *s, int kind) {
     return any;
 }
 
-/* updateAverageVelocity (dcollid.cpp:677-751) */
+static int32_t update_impact_zone_velocity(orc_solver *s, int rg_only);
+
+/* updateAverageVelocity (dcollid.cpp:677-751).  The RG counts are per body
+ * (every point of a body holds the same count), so they are cleared after
+ * all points have used them; collsnImpulse_RG is never cleared (the
+ * reference resets only collsn_num_RG, dcollid.cpp:728-734). */
+static void record_impulses(orc_solver *s, int any_hit) {
+    if (!s->P.record_impulses) return;
+    if (!s->last_imp) {
+        s->last_imp = (double *)calloc(3 * (size_t)s->nv, sizeof(double));
+        s->last_num = (int32_t *)calloc((size_t)s->nv, sizeof(int32_t));
+    }
+    if (any_hit < 0) {            /* start of a resolve */
+        memset(s->last_imp, 0, 3 * (size_t)s->nv * sizeof(double));
+        memset(s->last_num, 0, (size_t)s->nv * sizeof(int32_t));
+    } else if (any_hit) {         /* a pass with hits */
+        memcpy(s->last_imp, s->imp, 3 * (size_t)s->nv * sizeof(double));
+        memcpy(s->last_num, s->num, (size_t)s->nv * sizeof(int32_t));
+    }
+}
+
 static void update_average_velocity(orc_solver *s) {
     for (int32_t v = 0; v < s->nv; ++v) {
         if (s->fixed[v]) continue;
@@ -990,7 +1203,15 @@ static void update_average_velocity(orc_solver *s) {
             }
             s->num[v] = 0;
         }
+        int32_t b = s->nbody ? s->vbody[v] : -1;
+        if (b >= 0 && s->rg_num[b] > 0) {
+            s->has[v] = 1;
+            for (int k = 0; k < 3; ++k) s->av[3 * v + k] += s->rg_imp[3 * b + k] / s->rg_num[b];
+        }
     }
+    for (int32_t b = 0; b < s->nbody; ++b) s->rg_num[b] = 0;
+    /* updateImpactZoneVelocityForRG (dcollid.cpp:267-288) */
+    if (s->nbody && s->P.dt > 0.0) update_impact_zone_velocity(s, 1);
 }
 
 /* Sum of per-member terms in one of two orders:
@@ -1108,8 +1329,10 @@ static int zone_rigid(const orc_params *P, int32_t n, const int32_t *mem, const 
     return err;
 }
 
-/* updateImpactZoneVelocity (dcollid.cpp:290-310); returns #zones */
-static int32_t update_impact_zone_velocity(orc_solver *s) {
+/* updateImpactZoneVelocity (dcollid.cpp:290-310), every zone of two or more
+ * points; with rg_only updateImpactZoneVelocityForRG (dcollid.cpp:267-288),
+ * the zones holding a movable point (the bodies).  Returns #zones. */
+static int32_t update_impact_zone_velocity(orc_solver *s, int rg_only) {
     int32_t nz = 0;
     int32_t *mem = (int32_t *)malloc(sizeof(int32_t) * (size_t)s->nv);
     if (s->P.order_mode == ORC_ORDER_REFERENCE_TREE) {
@@ -1119,6 +1342,7 @@ static int32_t update_impact_zone_velocity(orc_solver *s) {
             for (int i = 0; i < np; ++i) {
                 int32_t pt = s->ev[3 * e + i];
                 if (s->sorted[pt] || s->uf_w[uf_find(s, pt)] == 1) continue;
+                if (rg_only && !is_movable(s, pt)) { s->sorted[pt] = 1; continue; }
                 int32_t head = uf_find(s, pt), n = 0;
                 for (int32_t q = head; q >= 0; q = s->uf_next[q]) { mem[n++] = q; s->sorted[q] = 1; }
                 if (zone_rigid(&s->P, n, mem, s->x_old, s->av, s->fixed, 0)) s->err = ORC_ERR_NONFINITE;
@@ -1141,6 +1365,7 @@ static int32_t update_impact_zone_velocity(orc_solver *s) {
         for (int32_t v = 0; v < s->nv; ++v) {
             int32_t r = root_of[v];
             if (done[r] || cnt[r] < 2) continue;
+            if (rg_only && !is_movable(s, v)) continue;
             done[r] = 1;
             if (zone_rigid(&s->P, cnt[r], &mem[start[r]], s->x_old, s->av, s->fixed, 64))
                 s->err = ORC_ERR_NONFINITE;
@@ -1152,28 +1377,76 @@ static int32_t update_impact_zone_velocity(orc_solver *s) {
     return nz;
 }
 
-/* detectDomainBoundaryCollision (dcollid.cpp:116-158); visiting a point
- * more than once is a no-op, so each point is visited once */
-static void domain_boundary(orc_solver *s) {
+/* detectDomainBoundaryCollision (dcollid.cpp:116-158): every point is
+ * visited once per element holding it, in element order (movable points
+ * are skipped).  A later visit can change a point again: when x_old lies
+ * outside the box on one axis, friction from an earlier visit can shorten
+ * the inward move so that the axis is touched on the next visit. */
+static void domain_visit(orc_solver *s, int32_t v) {
     double dt = s->P.dt, mu = s->P.friction;
-    for (int32_t v = 0; v < s->nv; ++v) {
-        double *avg = &s->av[3 * v];
-        double dv = 0;
-        for (int j = 0; j < 3; ++j) {
-            double cand = s->x_old[3 * v + j] + dt * avg[j];
-            double L = s->P.dom_lo[j], U = s->P.dom_hi[j];
-            if (cand <= L) {
-                s->has[v] = 1; dv = fabs(avg[j]); avg[j] = 0.0;
-                s->x[3 * v + j] = L + s->P.thickness;
-            } else if (cand >= U) {
-                s->has[v] = 1; dv = fabs(avg[j]); avg[j] = 0.0;
-                s->x[3 * v + j] = U - s->P.thickness;
+    double *avg = &s->av[3 * v];
+    double dv = 0;
+    for (int j = 0; j < 3; ++j) {
+        double cand = s->x_old[3 * v + j] + dt * avg[j];
+        double L = s->P.dom_lo[j], U = s->P.dom_hi[j];
+        if (cand <= L) {
+            s->has[v] = 1; dv = fabs(avg[j]); avg[j] = 0.0;
+            s->x[3 * v + j] = L + s->P.thickness;
+        } else if (cand >= U) {
+            s->has[v] = 1; dv = fabs(avg[j]); avg[j] = 0.0;
+            s->x[3 * v + j] = U - s->P.thickness;
+        }
+    }
+    double preVt = mag3(avg);
+    if (preVt > s->P.mach_eps)
+        for (int j = 0; j < 3; ++j) avg[j] *= smax(1.0 - mu * dv / preVt, 0.0);
+}
+
+static void domain_boundary(orc_solver *s) {
+    for (int32_t e = 0; e < s->ne; ++e) {
+        int np = elem_is_tri(s, e) ? 3 : 2;
+        for (int i = 0; i < np; ++i) {
+            int32_t v = s->ev[3 * e + i];
+            if (is_movable(s, v)) continue;
+            domain_visit(s, v);
+        }
+    }
+}
+
+/* updateFinalForRG (dcollid.cpp:626-675): the first point of each body (in
+ * element order) with has_collsn sets center_of_mass_velo to its avgVel and
+ * center_of_mass to avgVel * dt + mrg_com; mrg_com then follows
+ * center_of_mass.  The reference reads mrg_com of a body before its first
+ * write (an empty vector); here it starts at the center of mass given by
+ * orc_set_bodies. */
+static void update_final_for_rg(orc_solver *s) {
+    if (!s->nbody) return;
+    double dt = s->P.dt;
+    uint8_t *in_mrg = (uint8_t *)calloc((size_t)s->nbody, 1);
+    int8_t *visited = (int8_t *)malloc((size_t)s->nbody);   /* -1: no entry */
+    memset(visited, -1, (size_t)s->nbody);
+    for (int32_t e = 0; e < s->ne; ++e) {
+        int np = elem_is_tri(s, e) ? 3 : 2;
+        for (int i = 0; i < np; ++i) {
+            int32_t v = s->ev[3 * e + i];
+            if (!is_movable(s, v)) continue;
+            int32_t b = s->vbody[v];
+            if (s->has[v] && !in_mrg[b]) {
+                in_mrg[b] = 1;
+                for (int j = 0; j < 3; ++j) {
+                    s->bvel[3 * b + j] = s->av[3 * v + j];
+                    s->bcom[3 * b + j] = s->av[3 * v + j] * dt + s->mrg_com[3 * b + j];
+                }
+                visited[b] = 0;
+            }
+            if (visited[b] <= 0) {
+                for (int j = 0; j < 3; ++j) s->mrg_com[3 * b + j] = s->bcom[3 * b + j];
+                visited[b] = 1;
             }
         }
-        double preVt = mag3(avg);
-        if (preVt > MACH_EPS)
-            for (int j = 0; j < 3; ++j) avg[j] *= smax(1.0 - mu * dv / preVt, 0.0);
     }
+    free(in_mrg);
+    free(visited);
 }
 
 /* reduceSuperelastOnce (dcollid.cpp:486-560) */
@@ -1237,10 +1510,9 @@ int orc_resolve(orc_solver *s, orc_report *rep_out) {
     s->plog.n = 0;
     s->cur_pass = 0;
     double dt = s->P.dt;
-    /* per-step makeSet from assembleFromInterface (dcollid.cpp:1015-1031) */
-    for (int32_t v = 0; v < s->nv; ++v) {
-        s->uf_root[v] = v; s->uf_next[v] = -1; s->uf_tail[v] = v; s->uf_w[v] = 1;
-    }
+    /* per-step makeSet and body zones of assembleFromInterface
+     * (dcollid3d.cpp:44-45) */
+    make_set(s);
     /* computeAverageVelocity (dcollid.cpp:160-220) */
     for (int64_t i = 0; i < 3 * (int64_t)s->nv; ++i) {
         if (dt > ROUND_EPS) s->av[i] = (s->x[i] - s->x_old[i]) / dt;
@@ -1251,7 +1523,8 @@ int orc_resolve(orc_solver *s, orc_report *rep_out) {
     memcpy(s->x, s->x_old, sizeof(double) * 3 * (size_t)s->nv);
     /* detectProximity (dcollid.cpp:390-406) */
     s->zone_mode = 0;
-    run_pass(s, ORC_PASS_PROXIMITY);
+    record_impulses(s, -1);
+    record_impulses(s, run_pass(s, ORC_PASS_PROXIMITY));
     update_average_velocity(s);
     end_pass(s, ORC_PASS_PROXIMITY, 0, &rep);
     /* detectCollision (dcollid.cpp:430-468) */
@@ -1259,6 +1532,7 @@ int orc_resolve(orc_solver *s, orc_report *rep_out) {
     rep.has_collision = 0;
     while (is_collision && !s->err) {
         is_collision = run_pass(s, ORC_PASS_CCD);
+        record_impulses(s, is_collision);
         if (cd_count++ == 0 && is_collision) rep.has_collision = 1;
         update_average_velocity(s);
         end_pass(s, ORC_PASS_CCD, niter - 1, &rep);
@@ -1272,8 +1546,9 @@ int orc_resolve(orc_solver *s, orc_report *rep_out) {
         while (is_collision && !s->err) {
             if (zi >= s->P.max_zone_passes) { rep.status = ORC_WARN_ZONE_CAP; break; }
             is_collision = run_pass(s, ORC_PASS_ZONE);
+            record_impulses(s, is_collision);
             update_average_velocity(s);
-            rep.n_zones = update_impact_zone_velocity(s);
+            rep.n_zones = update_impact_zone_velocity(s, 0);
             end_pass(s, ORC_PASS_ZONE, zi, &rep);
             rep.n_zone_passes++;
             zi++;
@@ -1295,6 +1570,7 @@ int orc_resolve(orc_solver *s, orc_report *rep_out) {
     for (int32_t v = 0; v < s->nv; ++v)
         if (s->has[v])
             for (int k = 0; k < 3; ++k) s->vel[3 * v + k] = s->av[3 * v + k];
+    update_final_for_rg(s);
     if (s->err) rep.status = s->err;
     if (rep_out) *rep_out = rep;
     return rep.status;
@@ -1327,10 +1603,37 @@ int orc_zone_labels(orc_solver *s, int32_t *out) {
     return ORC_OK;
 }
 
+int orc_last_impulses(orc_solver *s, double *imp, int32_t *num) {
+    if (!s || !s->last_imp) return ORC_ERR_ARG;
+    if (imp) memcpy(imp, s->last_imp, 3 * (size_t)s->nv * sizeof(double));
+    if (num) memcpy(num, s->last_num, (size_t)s->nv * sizeof(int32_t));
+    return ORC_OK;
+}
+
+/* ---- test hooks ---- */
+int32_t orc_debug_pair(orc_solver *s, int32_t A, int32_t B, int32_t ccd, int32_t zone_mode) {
+    if (!s || !s->x || A < 0 || B < 0 || A >= s->ne || B >= s->ne) return -1;
+    imp_ctx C = {&s->P, s->fixed, s->nbody ? s->vbody : NULL, s->bmass, s->rg_imp, s->rg_num,
+                 s->av, s->x_old, s->imp, s->fric, s->num, 0};
+    int saved = s->zone_mode, ran = 0;
+    s->zone_mode = zone_mode;
+    int32_t mask = narrow_pair(s, &C, A, B, ccd, &ran);
+    s->zone_mode = saved;
+    if (C.err) return -1;
+    return mask;
+}
+
+int orc_debug_make_set(orc_solver *s) {
+    if (!s || !s->x) return ORC_ERR_ARG;
+    make_set(s);
+    return ORC_OK;
+}
+
 /* ---- unit entry points ---- */
 int orc_is_coplanar(const double *x_old, const double *avg_vel, double dt, int32_t math_mode,
                     double *roots) {
-    return is_coplanar(math_mode, (const double(*)[3])x_old, (const double(*)[3])avg_vel, dt, roots);
+    return is_coplanar(math_mode, DBL_EPSILON, (const double(*)[3])x_old,
+                       (const double(*)[3])avg_vel, dt, roots);
 }
 
 static int unit_subtest(const orc_params *p, const double *c, const double *x_old,
@@ -1342,7 +1645,7 @@ static int unit_subtest(const orc_params *p, const double *c, const double *x_ol
     memset(counts, 0, 4 * sizeof(int32_t));
     uint8_t fx[4] = {0, 0, 0, 0};
     if (is_fixed) memcpy(fx, is_fixed, 4);
-    imp_ctx C = {p, fx, avg_vel, x_old, imp, fric, counts, 0};
+    imp_ctx C = {p, fx, NULL, NULL, NULL, NULL, avg_vel, x_old, imp, fric, counts, 0};
     int32_t ids[4] = {0, 1, 2, 3};
     int hit = pt ? point_to_tri(&C, ids, (const double(*)[3])c, h, 0.0)
                  : edge_to_edge(&C, ids, (const double(*)[3])c, h, 0.0);
@@ -1370,4 +1673,21 @@ int orc_zone_rigid_update(const orc_params *p, int32_t n, const double *x_old, d
 double orc_math_acos(double x) { return orc_det_acos(x); }
 double orc_math_cos(double x) { return orc_det_cos(x); }
 double orc_math_sin(double x) { return orc_det_sin(x); }
-double orc_math_cbrt(double x) { return orc_det_cbrt(x); }
+double orc_math_pow_third(double x) { return orc_det_pow_third(x); }
+
+int orc_math_eval(int32_t fn, const double *x, double *y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: y[i] = orc_det_acos(x[i]); break;
+            case 1: y[i] = orc_det_cos(x[i]); break;
+            case 2: y[i] = orc_det_sin(x[i]); break;
+            case 3: y[i] = orc_det_pow_third(x[i]); break;
+            case 10: y[i] = acos(x[i]); break;
+            case 11: y[i] = cos(x[i]); break;
+            case 12: y[i] = sin(x[i]); break;
+            case 13: y[i] = pow(x[i], 1.0 / 3.0); break;
+            default: return ORC_ERR_ARG;
+        }
+    }
+    return ORC_OK;
+}

@@ -46,6 +46,9 @@ typedef struct orc_params {
     int32_t order_mode;          /* ORC_ORDER_* */
     int32_t math_mode;           /* ORC_MATH_* */
     int32_t log_pairs;           /* record every hit pair of every pass */
+    int32_t record_impulses;     /* keep collsnImpulse / collsn_num of the last pass with a hit */
+    double mach_eps;             /* FronTier MACH_EPS; DBL_EPSILON by default (the
+                                    value is not in the reference tree) */
 } orc_params;
 
 typedef struct orc_pass_info {
@@ -88,6 +91,17 @@ int orc_assemble(orc_solver *s, int32_t nv, int32_t nt, int32_t nb,
                  const double *tri_len0, const int32_t *bonds,
                  const double *bond_len0, const uint8_t *is_fixed);
 
+/* movable rigid bodies (STATE.is_movableRG, cdinit.cpp:201-202): vert_body[v]
+ * is the body of vertex v or -1; a body is one closed surface whose
+ * triangles all have movable vertices of that body, and bonds hold no
+ * movable vertex.  body_mass = total_mass, body_com = center_of_mass of
+ * each body (dcollid.cpp:641-650).  Call after orc_assemble. */
+int orc_set_bodies(orc_solver *s, int32_t nbody, const int32_t *vert_body,
+                   const double *body_mass, const double *body_com);
+/* center_of_mass and center_of_mass_velo of every body after the last
+ * resolve (updateFinalForRG, dcollid.cpp:626-675) */
+int orc_get_bodies(orc_solver *s, double *com, double *com_vel);
+
 /* per-vertex state in [nv][3] row-major */
 int orc_set_state(orc_solver *s, const double *x, const double *vel,
                   const uint8_t *has_collsn);
@@ -109,6 +123,21 @@ int orc_pair_log(orc_solver *s, int32_t *out4, int64_t cap);
 /* zone label per vertex = smallest vertex id in its impact zone */
 int orc_zone_labels(orc_solver *s, int32_t *out);
 
+/* summed collsnImpulse [nv][3] and collsn_num [nv] of the last pass with a
+ * hit in the last resolve, zero if none (record_impulses) */
+int orc_last_impulses(orc_solver *s, double *imp, int32_t *num);
+
+/* ---- test hooks ---- */
+/* overwrite avgVel (after orc_set_state) */
+int orc_set_avg_vel(orc_solver *s, const double *avg_vel);
+/* one pair through the narrow phase (isProximity / isCollision with the
+ * Moving* dispatch), with the impact-zone merge rule active when zone_mode
+ * is set; returns the hit mask of the sub-tests, impulses stay in the
+ * accumulators (not applied).  Zone labels come from orc_zone_labels. */
+int32_t orc_debug_pair(orc_solver *s, int32_t A, int32_t B, int32_t ccd, int32_t zone_mode);
+/* reset the union-find to the per-step state (makeSet + body zones) */
+int orc_debug_make_set(orc_solver *s);
+
 /* ---- unit entry points (single sub-tests on explicit data) ---- */
 /* isCoplanar: x_old[4][3], avg_vel[4][3] -> roots[4]; returns 0/1 */
 int orc_is_coplanar(const double *x_old, const double *avg_vel, double dt,
@@ -126,11 +155,17 @@ int orc_edge_to_edge(const orc_params *p, const double *c, const double *x_old,
  * sums members sequentially (reference), 64 in the canonical lane order */
 int orc_zone_rigid_update(const orc_params *p, int32_t n, const double *x_old,
                           double *avg_vel, const uint8_t *is_fixed, int32_t lanes);
-/* elementary functions of the deterministic math mode */
+/* elementary functions of the deterministic math mode (correctly rounded) */
 double orc_math_acos(double x);
 double orc_math_cos(double x);
 double orc_math_sin(double x);
-double orc_math_cbrt(double x);
+double orc_math_pow_third(double x);   /* pow(x, 1.0/3.0) */
+/* y[i] = f(x[i]); fn 0..3: acos, cos, sin, pow(x, 1.0/3.0) of the
+ * deterministic mode, 10..13: the same through the host libm */
+int orc_math_eval(int32_t fn, const double *x, double *y, int64_t n);
+/* calls in libm mode whose libm result was not the correctly rounded one,
+ * since the last reset (process-wide) */
+int64_t orc_libm_misrounded(int32_t reset);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,8 @@ class Params(C.Structure):
                 ("restitution", C.c_double), ("dom_lo", C.c_double * 3),
                 ("dom_hi", C.c_double * 3), ("max_ccd_passes", C.c_int32),
                 ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
-                ("order_mode", C.c_int32), ("math_mode", C.c_int32), ("log_pairs", C.c_int32)]
+                ("order_mode", C.c_int32), ("math_mode", C.c_int32), ("log_pairs", C.c_int32),
+                ("record_impulses", C.c_int32), ("mach_eps", C.c_double)]
 
 
 class PassInfo(C.Structure):
@@ -85,13 +86,23 @@ def lib():
         L.orc_pair_log_count.restype = i64
         L.orc_pair_log.argtypes = [C.c_void_p, P(i32), i64]
         L.orc_zone_labels.argtypes = [C.c_void_p, P(i32)]
+        L.orc_set_bodies.argtypes = [C.c_void_p, i32, P(i32), P(d), P(d)]
+        L.orc_get_bodies.argtypes = [C.c_void_p, P(d), P(d)]
+        L.orc_set_avg_vel.argtypes = [C.c_void_p, P(d)]
+        L.orc_last_impulses.argtypes = [C.c_void_p, P(d), P(i32)]
+        L.orc_debug_pair.argtypes = [C.c_void_p, i32, i32, i32, i32]
+        L.orc_debug_pair.restype = i32
+        L.orc_debug_make_set.argtypes = [C.c_void_p]
         L.orc_is_coplanar.argtypes = [P(d), P(d), d, i32, P(d)]
         for fn in (L.orc_point_to_tri, L.orc_edge_to_edge):
             fn.argtypes = [P(Params), P(d), P(d), P(d), P(u8), d, P(d), P(d), P(i32)]
         L.orc_zone_rigid_update.argtypes = [P(Params), i32, P(d), P(d), P(u8), i32]
-        for fn in (L.orc_math_acos, L.orc_math_cos, L.orc_math_sin, L.orc_math_cbrt):
+        for fn in (L.orc_math_acos, L.orc_math_cos, L.orc_math_sin, L.orc_math_pow_third):
             fn.argtypes = [d]
             fn.restype = d
+        L.orc_math_eval.argtypes = [i32, P(d), P(d), i64]
+        L.orc_libm_misrounded.argtypes = [i32]
+        L.orc_libm_misrounded.restype = i64
         _lib = L
     return _lib
 
@@ -145,6 +156,15 @@ class OracleSolver:
         if rc:
             raise ValueError(f"orc_assemble failed: {rc}")
         self.set_state(scene.x, scene.vel)
+        self.nbody = getattr(scene, "nbody", 0)
+        if self.nbody:
+            vb = np.ascontiguousarray(scene.vert_body, np.int32)
+            bm = np.ascontiguousarray(scene.body_mass, np.float64)
+            bc = np.ascontiguousarray(scene.body_com, np.float64)
+            rc = self.L.orc_set_bodies(self.h, self.nbody, _ptr(vb, C.c_int32), _ptr(bm, C.c_double),
+                                       _ptr(bc, C.c_double))
+            if rc:
+                raise ValueError(f"orc_set_bodies failed: {rc}")
 
     def close(self):
         if self.h:
@@ -203,8 +223,52 @@ class OracleSolver:
         self.L.orc_zone_labels(self.h, _ptr(out, C.c_int32))
         return out
 
+    def bodies(self):
+        """(center_of_mass, center_of_mass_velo) of the movable bodies"""
+        com = np.zeros((max(self.nbody, 1), 3))
+        vel = np.zeros((max(self.nbody, 1), 3))
+        self.L.orc_get_bodies(self.h, _ptr(com, C.c_double), _ptr(vel, C.c_double))
+        return com[:self.nbody], vel[:self.nbody]
+
+    def last_impulses(self):
+        imp, num = np.empty((self.nv, 3)), np.empty(self.nv, np.int32)
+        if self.L.orc_last_impulses(self.h, _ptr(imp, C.c_double), _ptr(num, C.c_int32)):
+            raise ValueError("record_impulses was not set")
+        return imp, num
+
+    # -- test hooks --
+    def set_avg_vel(self, av):
+        av = np.ascontiguousarray(av, np.float64)
+        self.L.orc_set_avg_vel(self.h, _ptr(av, C.c_double))
+
+    def debug_make_set(self):
+        self.L.orc_debug_make_set(self.h)
+
+    def debug_pair(self, A, B, ccd=1, zone_mode=0):
+        """hit mask of one pair through the narrow phase (-1: error)"""
+        return self.L.orc_debug_pair(self.h, A, B, ccd, zone_mode)
+
 
 # ---- unit-level helpers ----
+MATH_FUNCS = {"acos": 0, "cos": 1, "sin": 2, "pow_third": 3}
+
+
+def libm_misrounded(reset=False):
+    """libm-mode calls since the last reset whose libm result was not the
+    correctly rounded one"""
+    return lib().orc_libm_misrounded(1 if reset else 0)
+
+
+def math_eval(name, x, libm=False):
+    """vectorised deterministic-mode (or host libm) acos / cos / sin / pow(x, 1/3)"""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    fn = MATH_FUNCS[name] + (10 if libm else 0)
+    rc = lib().orc_math_eval(fn, _ptr(x, C.c_double), _ptr(y, C.c_double), x.size)
+    if rc:
+        raise ValueError(name)
+    return y
+
 def is_coplanar(x_old, avg_vel, dt, math_mode=MATH_DET):
     xo = np.ascontiguousarray(x_old, np.float64).reshape(4, 3)
     av = np.ascontiguousarray(avg_vel, np.float64).reshape(4, 3)

@@ -26,8 +26,12 @@ def sorted_log(log):
 
 
 def run_parity(scene, steps, check_every=1, compare_zones=True, verbose=False, gpu_kw=None,
-               **kw):
-    gpu = solver_for_scene(scene, log_pairs=1, **(gpu_kw or {}), **kw)
+               gpu_log_pairs=1, **kw):
+    """gpu_log_pairs=0 runs the library as shipped (no parity taps: the pair
+    stream is not carried between incremental passes and their candidate and
+    pair counts are -1); the hit counts, zones and the full state are still
+    compared every step (check_every only applies with the taps on)."""
+    gpu = solver_for_scene(scene, log_pairs=gpu_log_pairs, **(gpu_kw or {}), **kw)
     orc = O.OracleSolver(scene, log_pairs=1, order_mode=O.ORDER_CANONICAL,
                          math_mode=O.MATH_DET, **kw)
     stats = dict(steps=0, hit_pairs=0, zone_steps=0, strain_sweeps=0)
@@ -40,20 +44,29 @@ def run_parity(scene, steps, check_every=1, compare_zones=True, verbose=False, g
                   flush=True) if verbose else None
             assert rg.status == ro.status, (step, rg.status, ro.status)
             dg, do = rg.as_dict(), ro.as_dict()
+            taps = gpu_log_pairs != 0
             for k in REPORT_KEYS:
+                if not taps and k in ("candidates", "narrow_pairs") and dg[k] == -1:
+                    continue
                 assert dg[k] == do[k], (step, k, dg[k], do[k])
             pg, po = gpu.passes(), orc.passes()
             assert len(pg) == len(po), (step, len(pg), len(po))
             for a, b in zip(pg, po):
                 for k in PASS_KEYS:
+                    if not taps and k in ("candidates", "narrow_pairs") and a[k] == -1:
+                        continue
                     assert a[k] == b[k], (step, k, a, b)
-            lg, lo = sorted_log(gpu.pairLog()), sorted_log(orc.pair_log())
-            assert lg.shape == lo.shape, (step, lg.shape, lo.shape)
-            assert np.array_equal(lg, lo), (step, "pair log differs")
+            if taps:
+                lg, lo = sorted_log(gpu.pairLog()), sorted_log(orc.pair_log())
+                assert lg.shape == lo.shape, (step, lg.shape, lo.shape)
+                assert np.array_equal(lg, lo), (step, "pair log differs")
             if compare_zones and ro.n_zone_passes:
                 assert np.array_equal(gpu.zoneLabels(), orc.zone_labels()), (step, "zones")
                 stats["zone_steps"] += 1
-            if (step + 1) % check_every == 0 or step == steps - 1:
+            if not taps or (step + 1) % check_every == 0 or step == steps - 1:
+                if getattr(scene, "nbody", 0):
+                    (cg, vg), (co, vo) = gpu.getBodies(), orc.bodies()
+                    assert np.array_equal(cg, co) and np.array_equal(vg, vo), (step, "bodies")
                 sg, so = gpu.getState(), orc.get_state()
                 for k in ("x", "vel", "avg_vel", "x_old", "has_collsn"):
                     if not np.array_equal(sg[k], so[k]):

@@ -104,3 +104,223 @@ def test_cpp_host_api_matches_oracle(tmp_path):
         hits += orc.step().hit_pairs
     assert hits > 0
     assert np.array_equal(orc.get_state()["x"], x_gpu)
+
+
+# ---------------------------------------------------------------------------
+def test_device_math_is_bitwise_the_oracles():
+    """The device's acos / cos / sin / pow(x, 1/3) (cs_math_eval) equal the
+    oracle's correctly rounded functions bit for bit on 10^6 arguments each
+    (the oracle's are pinned against 70-digit arithmetic in test_oracle_math)."""
+    from collision_amd.solver import math_eval
+    from oracle import oracle as O
+    rng = np.random.default_rng(21)
+    n = 1 << 20
+    cases = {"acos": np.concatenate([rng.uniform(-1, 1, n), [-1.0, 1.0, 0.0, 0.5, -0.5]]),
+             "cos": np.concatenate([rng.uniform(-2.2, 2.2, n), rng.uniform(-40, 40, n // 8)]),
+             "sin": np.concatenate([rng.uniform(-2.2, 2.2, n), rng.uniform(-1e-3, 1e-3, n // 8)]),
+             "pow_third": np.concatenate([10.0 ** rng.uniform(-300, 300, n), [0.0, 8.0, 27.0]])}
+    for name, x in cases.items():
+        g, o = math_eval(name, x), O.math_eval(name, x)
+        assert np.array_equal(g.view(np.uint64), o.view(np.uint64)), name
+
+
+@pytest.mark.parametrize("maker", [S.deck_ball_plane, S.deck_box_boundary, S.deck_string_slide])
+def test_movable_rigid_body_decks(maker):
+    """Movable rigid bodies (is_movableRG): body zones, RG impulses spread
+    over the body, rigid averaging in every apply, centre-of-mass update;
+    restated from in-ball_plane, in-box_boundary and in-string_slide."""
+    sc = maker()
+    st = run_parity(sc, steps=min(sc.steps, 150), verbose=True)
+    assert st["hit_pairs"] > 0
+
+
+def test_domain_box_fires_on_gpu():
+    """Strings driven into a tight domain box, one starting outside it on one
+    axis: the domain update repeats once per element of a point
+    (dcollid.cpp:119-156)."""
+    st = run_parity(S.domain_wall_strings(), steps=40, verbose=True)
+    assert st["steps"] == 40
+
+
+def test_nonfinite_average_velocity_stops_before_any_pass():
+    """computeAverageVelocity calls clean_up(ERROR) on NaN (dcollid.cpp:178-184):
+    both stop with the non-finite status before the proximity pass."""
+    from collision_amd.solver import CollisionError, solver_for_scene
+    from oracle import oracle as O
+    sc = S.cloth_over_sphere(n=8, level=1)
+    sc.vel[3, 1] = np.nan
+    orc = O.OracleSolver(sc)
+    ro = orc.step()
+    orc.close()
+    gpu = solver_for_scene(sc)
+    with pytest.raises(CollisionError) as ei:
+        gpu.step()
+    assert ei.value.status == ro.status == 2
+    assert gpu.L.cs_pass_count(gpu.h) == 0
+    gpu.close()
+
+
+def test_zone_cap_warning_matches():
+    """With the impact-zone loop capped at one pass both report
+    CS_WARN_ZONE_CAP in the same steps and finish those steps identically."""
+    sc = S.config0()
+    sc.x = sc.x + sc.vel * (sc.dt * 236)
+    st = run_parity(sc, steps=64, max_zone_passes=1, verbose=True)
+    assert st["zone_steps"] >= 1
+
+
+@pytest.mark.parametrize("full", [0, 1], ids=["incremental", "full_passes"])
+def test_shipped_mode_without_parity_taps(full):
+    """The library as shipped (log_pairs = 0: no pair stream carried between
+    incremental passes, their candidate counts -1) against the oracle: hit
+    counts, zones and the complete state after every step."""
+    sc = S.combined_scene(n=96, nstrings=60, steps=60)
+    st = run_parity(sc, steps=60, gpu_log_pairs=0, gpu_kw=dict(force_full_passes=full))
+    assert st["hit_pairs"] > 0 and st["zone_steps"] > 0
+
+
+def test_shipped_mode_without_parity_taps_config0():
+    sc = S.config0()
+    sc.x = sc.x + sc.vel * (sc.dt * 236)
+    st = run_parity(sc, steps=64, gpu_log_pairs=0)
+    assert st["hit_pairs"] > 0
+
+
+def _golden_run(name):
+    import json
+    import os
+
+    from collision_amd.solver import solver_for_scene
+    from tests.golden.make_step_golden import PASS_KEYS, REPORT_KEYS, WORKLOADS, log_hash, state_hash
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"steps_{name}.json")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} not generated")
+    gold = json.load(open(path))
+    sc = WORKLOADS[name][0]()
+    assert (sc.nv, sc.nt, sc.nb) == (gold["nv"], gold["nt"], gold["nb"])
+    gpu = solver_for_scene(sc, log_pairs=1)
+    try:
+        for k, rec in enumerate(gold["records"]):
+            r = gpu.step().as_dict()
+            print(f"[golden] {name} step {k} passes {r['n_passes']} hits {r['hit_pairs']}", flush=True)
+            assert {key: int(r[key]) for key in REPORT_KEYS} == rec["report"], k
+            assert [{key: p[key] for key in PASS_KEYS} for p in gpu.passes()] == rec["passes"], k
+            assert log_hash(gpu.pairLog()) == rec["log"], k
+            assert state_hash(gpu.getState()) == rec["state"], k
+    finally:
+        gpu.close()
+    return gold
+
+
+def test_config1_full_size_against_golden():
+    """Config 1 at full size (256 x 256 cloth, 100 steps): every step's
+    counts, hit pairs and complete state bitwise equal the oracle's
+    (committed golden records, tests/golden/make_step_golden.py)."""
+    g = _golden_run("config1_256")
+    assert sum(r["report"]["hit_pairs"] for r in g["records"]) > 10000
+
+
+def test_config4_256_against_golden():
+    g = _golden_run("config4_256")
+    assert sum(r["report"]["n_zone_passes"] for r in g["records"]) > 0
+
+
+def test_config4_full_size_incremental_equals_full_passes():
+    """Full-size config 4 (1024 x 1024 cloth, 2000 strings), 15 contact steps
+    from 8 steps before the first box contact: status OK, no non-finite value,
+    and the incremental passes give bitwise the state of all-full passes."""
+    from collision_amd.solver import solver_for_scene
+    sc = S.config4()
+    sc.x = sc.x + sc.vel * (sc.dt * 2)
+    a = solver_for_scene(sc, force_full_passes=0)
+    b = solver_for_scene(sc, force_full_passes=1)
+    hits = 0
+    try:
+        for k in range(15):
+            ra, rb = a.step(), b.step()
+            print(f"[c4full] step {k} passes {ra.n_passes}/{rb.n_passes} hits {ra.hit_pairs}",
+                  flush=True)
+            assert ra.status == 0 and rb.status == 0
+            for key in ("n_passes", "n_ccd_passes", "n_zone_passes", "hit_pairs", "hit_subtests",
+                        "strain_sweeps", "n_zones"):
+                assert getattr(ra, key) == getattr(rb, key), (k, key)
+            hits += ra.hit_pairs
+        sa, sb = a.getState(), b.getState()
+        for key in ("x", "vel", "avg_vel", "has_collsn"):
+            assert np.array_equal(sa[key], sb[key]), key
+        assert np.all(np.isfinite(sa["x"])) and np.all(np.isfinite(sa["vel"]))
+        assert hits > 0
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("maker,n", [(lambda: S.combined_scene(n=96, nstrings=60), 20),
+                                     (S.deck_box_boundary, 40)], ids=["config4_96", "box_boundary"])
+def test_checkpoint_restart_is_bitwise(tmp_path, maker, n):
+    """N + N steps equal N steps, cs_save, cs_load into a fresh handle, N
+    steps: every per-vertex array and the body state, bit for bit."""
+    from collision_amd.solver import solver_for_scene
+    sc = maker()
+    a = solver_for_scene(sc)
+    for _ in range(n):
+        a.step()
+    ck = tmp_path / "state.ckpt"
+    a.save(ck)
+    for _ in range(n):
+        a.step()
+    b = solver_for_scene(sc, friction=0.77)     # parameters come from the checkpoint
+    b.load(ck)
+    assert b.params.friction == sc.friction
+    for _ in range(n):
+        b.step()
+    sa, sb = a.getState(), b.getState()
+    for k in ("x", "vel", "avg_vel", "x_old", "has_collsn"):
+        assert np.array_equal(sa[k], sb[k]), k
+    (ca, va), (cb, vb) = a.getBodies(), b.getBodies()
+    assert np.array_equal(ca, cb) and np.array_equal(va, vb)
+    a.close()
+    b.close()
+
+
+def test_vtk_dump_of_the_last_pass(tmp_path):
+    """cs_write_vtk (vtkplotVectorSurface, vtk.cpp:14-93): the summed impulses
+    of each step's last pass with a hit equal the oracle's bit for bit, and
+    the .vtp file holds
+    the current points, one polygon per element, the impulses and red cells
+    exactly where a point had a hit."""
+    import xml.etree.ElementTree as ET
+
+    from collision_amd.solver import solver_for_scene
+    from oracle import oracle as O
+    sc = S.config0()
+    sc.x = sc.x + sc.vel * (sc.dt * 246)
+    gpu = solver_for_scene(sc, record_impulses=1)
+    orc = O.OracleSolver(sc, record_impulses=1)
+    for _ in range(6):
+        gpu.step()
+        orc.step()
+        ig, ng = gpu.lastImpulses()
+        io, no = orc.last_impulses()
+        assert np.array_equal(ig, io) and np.array_equal(ng, no)
+    assert ng.sum() > 0
+    path = tmp_path / "surf.vtp"
+    gpu.writeVtk(path)
+    piece = ET.parse(path).getroot().find("PolyData/Piece")
+    assert int(piece.get("NumberOfPoints")) == sc.nv and int(piece.get("NumberOfPolys")) == sc.nt
+    arrays = {d.get("Name"): np.array(d.text.split(), float) for d in piece.iter("DataArray")}
+    order = []
+    for t in sc.tris:
+        for v in t:
+            if v not in order:
+                order.append(v)
+    order = np.array(order)
+    x = gpu.getState()["x"]
+    np.testing.assert_array_equal(arrays["Points"].reshape(-1, 3), x[order].astype(np.float32))
+    np.testing.assert_array_equal(arrays["CollsnImpulse"].reshape(-1, 3),
+                                  ig[order].astype(np.float32))
+    red = (ng[sc.tris] > 0).any(axis=1)
+    col = arrays["CollsnRegion"].reshape(-1, 3)
+    assert np.array_equal(col[:, 0] == 255, red) and np.array_equal(col[:, 1] == 255, ~red)
+    gpu.close()
+    orc.close()
