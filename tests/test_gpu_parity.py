@@ -308,7 +308,9 @@ def test_vtk_dump_of_the_last_pass(tmp_path):
     gpu.writeVtk(path)
     piece = ET.parse(path).getroot().find("PolyData/Piece")
     assert int(piece.get("NumberOfPoints")) == sc.nv and int(piece.get("NumberOfPolys")) == sc.nt
-    arrays = {d.get("Name"): np.array(d.text.split(), float) for d in piece.iter("DataArray")}
+    # 9 significant digits give back every float32 value exactly
+    arrays = {d.get("Name"): np.array(d.text.split(), np.float64).astype(
+        np.float32 if d.get("type") == "Float32" else np.float64) for d in piece.iter("DataArray")}
     order = []
     for t in sc.tris:
         for v in t:
