@@ -115,6 +115,7 @@ def lib(path: str = LIB_PATH):
         "cs_pair_log_count": ([vp], i64),
         "cs_pair_log": ([vp, P(i32), i64], C.c_int),
         "cs_zone_labels": ([vp, P(i32)], C.c_int),
+        "cs_zone_labels_noalloc": ([vp, P(i32)], C.c_int),
         "cs_get_bodies": ([vp, P(d), P(d)], C.c_int),
         "cs_math_eval": ([i32, i32, P(d), P(d), i64], C.c_int),
         "cs_last_impulses": ([vp, P(d), P(i32)], C.c_int),

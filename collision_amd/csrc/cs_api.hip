@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,10 +24,36 @@
 
 using namespace cs;
 
+// Staging memory of cs_zone_labels_noalloc: a fixed device array instead of
+// a hipMalloc per call.  Each device holds its own copy for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress
+// returns for that device (the array's name is a host-side stand-in).
+constexpr int NOALLOC_MAX_VERTICES = 1 << 22;  // 16 MiB of int32 labels
+static __device__ __align__(256) int32_t scratch[NOALLOC_MAX_VERTICES];
+
 namespace {
 
 constexpr int COUNTER_INTS = CNT_INTS;
 std::string g_create_error;
+
+// per-device address of `scratch`, looked up once per device in cs_create
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int32_t* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// with `device` current
+void init_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_done[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? (int32_t*)p : nullptr;
+  g_scratch_done[device] = true;
+}
 
 // sizes of the previous pass's candidate stream, hits and hit pairs
 struct PrevPass {
@@ -664,6 +691,7 @@ int cs_create(cs_solver** out, const cs_params* p) {
     delete h;
     return fail(nullptr, CS_ERR_HIP, "HIP stream / pinned buffer creation failed");
   }
+  init_scratch(h->P.device);  // the staging array of cs_zone_labels_noalloc
   *out = h;
   return CS_OK;
 }
@@ -1382,6 +1410,27 @@ int cs_load(cs_solver* h, const char* path) {
   h->state_set = true;
   h->prev = PrevPass();
   h->seed = SeedPairs();
+  return CS_OK;
+}
+
+int cs_zone_labels_noalloc(cs_solver* h, int32_t* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!out) return CS_ERR_ARG;
+  const int dev = h->P.device;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_done[dev])
+    return fail(h, CS_ERR_INTERNAL, "no staging array was set up for this device");
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+    return fail(h, CS_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                   hipGetErrorString(g_scratch_err[dev]));
+  if (h->S.nv > NOALLOC_MAX_VERTICES)
+    return fail(h, CS_ERR_UNSUPPORTED,
+                "cs_zone_labels_noalloc handles at most 4194304 vertices; use cs_zone_labels");
+  int32_t* d = g_scratch_ptr[dev];  // every label is written, nothing to clear
+  launch_zone_labels(h->st, h->S, d);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, d, sizeof(int32_t) * h->S.nv, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
   return CS_OK;
 }
 

@@ -185,6 +185,13 @@ int64_t cs_pair_log_count(cs_solver *h);
 int cs_pair_log(cs_solver *h, int32_t *out4, int64_t cap);
 /* zone label per vertex = smallest vertex id of its impact zone */
 int cs_zone_labels(cs_solver *h, int32_t *out);
+/* cs_zone_labels without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The labels are staged
+ * in a fixed 16 MiB device array per GPU, so at most 4194304 vertices
+ * (CS_ERR_UNSUPPORTED beyond).  All handles on one GPU share that array:
+ * calls that overlap on one GPU race. */
+int cs_zone_labels_noalloc(cs_solver *h, int32_t *out);
 
 /* y[i] = f(x[i]) on the device with the collision step's own elementary
  * functions (fn 0 acos, 1 cos, 2 sin, 3 pow(x, 1.0/3.0); correctly rounded),
