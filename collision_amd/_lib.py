@@ -38,7 +38,8 @@ class Params(C.Structure):
                 ("max_zone_passes", C.c_int32), ("max_strain_sweeps", C.c_int32),
                 ("log_pairs", C.c_int32), ("device", C.c_int32),
                 ("force_full_passes", C.c_int32), ("pass_timing", C.c_int32),
-                ("record_impulses", C.c_int32)]
+                ("record_impulses", C.c_int32), ("precision", C.c_int32),
+                ("learned", C.c_int32), ("precision_stats", C.c_int32), ("pad1", C.c_int32)]
 
 
 class MeshView(C.Structure):
@@ -68,10 +69,25 @@ class StepReport(C.Structure):
                 ("hit_pairs", C.c_int64), ("hit_subtests", C.c_int64),
                 ("ms_total", C.c_double), ("ms_broad", C.c_double), ("ms_narrow", C.c_double),
                 ("ms_response", C.c_double), ("ms_zone", C.c_double), ("ms_strain", C.c_double),
-                ("strain_slots", C.c_int64)]
+                ("strain_slots", C.c_int64), ("ms_strain_kernel", C.c_double),
+                ("host_syncs", C.c_int64), ("mx_screened", C.c_int64), ("mx_rechecked", C.c_int64),
+                ("mx_err_dist", C.c_double), ("mx_err_param", C.c_double),
+                ("mx_bound_ratio", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [("scored", C.c_int64), ("exists", C.c_int64), ("predicted_exists", C.c_int64),
+                ("agree", C.c_int64), ("newton_roots", C.c_int64), ("newton_iters", C.c_int64),
+                ("newton_iters_mid", C.c_int64), ("max_root_err", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MLP_PARAMS = 19970
 
 
 def declared_symbols(header: str = HEADER):
@@ -122,6 +138,17 @@ def lib(path: str = LIB_PATH):
         "cs_write_vtk": ([vp, C.c_char_p], C.c_int),
         "cs_save": ([vp, C.c_char_p], C.c_int),
         "cs_load": ([vp, C.c_char_p], C.c_int),
+        "cs_learned_set_weights": ([vp, P(C.c_float)], C.c_int),
+        "cs_learned_get_weights": ([vp, P(C.c_float)], C.c_int),
+        "cs_learned_init": ([vp, C.c_uint64], C.c_int),
+        "cs_learned_infer": ([vp, P(C.c_float), P(C.c_float), i64], C.c_int),
+        "cs_learned_grad": ([vp, P(C.c_float), P(C.c_float), P(C.c_float), i64, P(C.c_float),
+                             P(C.c_float)], C.c_int),
+        "cs_learned_collect": ([vp, i64, i32], C.c_int),
+        "cs_learned_dataset_size": ([vp], i64),
+        "cs_learned_dataset": ([vp, P(C.c_float), P(C.c_float), P(C.c_float), i64], C.c_int),
+        "cs_learned_train": ([vp, i32, i32, C.c_float, C.c_uint64, P(C.c_float)], C.c_int),
+        "cs_learned_get_stats": ([vp, P(LearnedStats)], C.c_int),
         "cs_last_error": ([vp], C.c_char_p),
         "cs_version": ([], C.c_char_p),
     }

@@ -94,6 +94,9 @@ struct cs_solver {
   size_t ev_used = 0;
   PrevPass prev;
   SeedPairs seed;
+  int64_t syncs = 0;  // host synchronisations of the current resolve
+  LearnedState L;           // time-of-impact proposer (learned form)
+  cs_learned_stats lstats{};  // its statistics since the last cs_learned_get_stats
 };
 
 namespace {
@@ -240,6 +243,7 @@ int read_counters(cs_solver* h) {
   HIPCHK(h, hipMemcpyAsync(h->h_counters, h->S.counters, sizeof(int) * COUNTER_INTS,
                            hipMemcpyDeviceToHost, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
+  h->syncs++;
   return CS_OK;
 }
 
@@ -258,6 +262,7 @@ unsigned long long c64(const cs_solver* h, int slot) {
 int set_counter(cs_solver* h, int slot, int value) {
   HIPCHK(h, hipMemcpyAsync(h->S.counters + slot, &value, sizeof(int), hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));  // `value` is a host stack variable
+  h->syncs++;
   return CS_OK;
 }
 
@@ -284,6 +289,24 @@ hipError_t grow_hit_buffers(cs_solver* h, int nhits, int nhp, int keep_hits, int
     S.hp_cap = cap;
   }
   return hipSuccess;
+}
+
+// The narrow phase over a pair stream.  In the learned form a CCD pass first
+// scores the stream's sub-tests with the network and tests its pairs in
+// order of decreasing score (the results do not depend on the order: hits
+// are sorted by key before use and zone merges commute).
+hipError_t narrow_call(cs_solver* h, NarrowParams NP, const int2* pairs, int npairs,
+                       const int* npairs_dev, int cap) {
+  if (npairs <= 0) return hipSuccess;
+  if (NP.ccd && h->P.learned) {
+    const uint32_t* perm = nullptr;
+    CS_HIP_RET(launch_learned_score(h->st, h->S, h->L, pairs, npairs, npairs_dev, cap, NP.dt,
+                                    NP.thickness > 0 ? NP.thickness : 1.0, &perm));
+    NP.lseed = h->L.seeds;
+    NP.perm = perm;
+  }
+  launch_narrow(h->st, h->S, NP, pairs, npairs, npairs_dev, cap);
+  return hipGetLastError();
 }
 
 // One broad + narrow + response pass.  *any_hit = some pair reported contact.
@@ -331,14 +354,25 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
                            (1u << CNT_HIT_SUB) | (1u << CNT_QPAIRS) | (1u << CNT_DIRTY) |
                            (0x3fu << CNT_CAND64) | (1u << CNT_DRANGES) |
                            (1u << (CNT_VLIST + (pass_id & 1))));
-  NarrowParams NP;
+  NarrowParams NP{};
   NP.dt = P.dt; NP.thickness = P.thickness; NP.round_eps = P.round_eps;
   NP.k = P.spring_k; NP.m = P.mass; NP.lambda = P.friction; NP.cr = P.restitution;
   NP.vbody = S.vbody; NP.bmass = S.bmass;
+  NP.mixed = P.precision == 1;
+  NP.mixed_stats = P.precision == 1 && P.precision_stats != 0;
   NP.ccd = ccd;
   NP.zone_mode = (kind == CS_PASS_ZONE);
   NP.log_pairs = P.log_pairs;
   NP.pass_id = pass_id;
+  if (ccd && h->L.ds_cap > 0) {  // dataset collection (cs_learned_collect)
+    NP.dsx = h->L.dsx;
+    NP.dse = h->L.dse;
+    NP.dst = h->L.dst;
+    NP.ds_count = h->L.ds_count;
+    NP.ds_cap = h->L.ds_cap;
+    NP.ds_stride = h->L.ds_stride;
+    NP.ds_salt = h->L.ds_salt++;
+  }
   int npairs = 0, carried_hits = 0, carried_hp = 0, carried_sub = 0, carried_pairs = 0;
   unsigned long long cand = 0;
   int nq = 0;
@@ -387,7 +421,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     mark_pass(h, &T.e1, 2);
     for (;;) {
       mark_pass(h, &T.n0, 1);
-      launch_narrow(h->st, S, NP, stream, npairs, stream_dev, S.pair_cap);
+      HIPCHK(h, narrow_call(h, NP, stream, npairs, stream_dev, S.pair_cap));
       mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
@@ -448,7 +482,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     narrow_n = npairs;
     for (;;) {
       mark_pass(h, &T.n0, 1);
-      launch_narrow(h->st, S, NP, S.pairs, npairs, nullptr, 0);
+      HIPCHK(h, narrow_call(h, NP, S.pairs, npairs, nullptr, 0));
       mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
@@ -491,14 +525,14 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
     const int guess = std::min(S.qpair_cap, std::max(4096, 2 * prev.nq + 1024));
     for (int attempt = 0;; ++attempt) {
       mark_pass(h, &T.n0, 1);
-      launch_narrow(h->st, S, NP, S.qpairs, guess, S.counters + CNT_QPAIRS, S.qpair_cap);
+      HIPCHK(h, narrow_call(h, NP, S.qpairs, guess, S.counters + CNT_QPAIRS, S.qpair_cap));
       mark_pass(h, &T.n1, 1);
       HIPCHK(h, hipGetLastError());
       int rc = read_counters(h);
       if (rc) return rc;
       nq = h->h_counters[CNT_QPAIRS];
       if (nq > guess && nq <= S.qpair_cap) {  // the rest of the stream
-        launch_narrow(h->st, S, NP, S.qpairs + guess, nq - guess, nullptr, 0);
+        HIPCHK(h, narrow_call(h, NP, S.qpairs + guess, nq - guess, nullptr, 0));
         mark_pass(h, &T.n1, 1);
         HIPCHK(h, hipGetLastError());
         if ((rc = read_counters(h))) return rc;
@@ -701,6 +735,7 @@ int cs_destroy(cs_solver* h) {
   (void)hipSetDevice(h->P.device);
   if (h->st) (void)hipStreamSynchronize(h->st);
   free_dev(h->S);
+  learned_free(h->L);
   for (auto e : h->events) (void)hipEventDestroy(e);
   if (h->h_counters) (void)hipHostFree(h->h_counters);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -713,6 +748,8 @@ int cs_set_params(cs_solver* h, const cs_params* p) {
   if (p->device != h->P.device) return fail(h, CS_ERR_ARG, "device cannot change after create");
   if (!(p->dt >= 0) || !(p->thickness >= 0) || !(p->round_eps >= 0))
     return fail(h, CS_ERR_ARG, "dt, thickness and round_eps must be >= 0");
+  if (p->precision < 0 || p->precision > 1 || p->learned < 0 || p->learned > 1)
+    return fail(h, CS_ERR_ARG, "precision and learned must be 0 or 1");
   h->P = *p;
   return CS_OK;
 }
@@ -1082,6 +1119,8 @@ int cs_predict(cs_solver* h) {
 int cs_resolve(cs_solver* h, cs_step_report* report) {
   int rc = check_ready(h);
   if (rc) return rc;
+  if (h->P.learned && !h->L.ready)
+    return fail(h, CS_ERR_ARG, "the learned form needs weights (cs_learned_init / _set_weights)");
   DevState& S = h->S;
   const cs_params& P = h->P;
   cs_step_report R{};
@@ -1090,8 +1129,10 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   h->plog_rows = 0;
   h->ev_used = 0;
   h->has_collision = 0;
+  h->syncs = 0;
   h->prev = PrevPass();
   hipEvent_t t0, t_strain0, t_end;
+  std::vector<hipEvent_t> strain_marks;
   mark(h, &t0);
   // the dirty-range marks of the last pass are cleared while their count is
   // still in the counter block
@@ -1147,7 +1188,8 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
     launch_final_position(h->st, S, P.dt);
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_FLAG, 0, sizeof(int) * 4, h->st));
     HIPCHK(h, hipMemsetAsync(S.counters + CNT_STRAIN_LEVELS, 0, sizeof(int) * 2, h->st));
-    HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps));
+    HIPCHK(h, strain_limit(h->st, S, P.dt, P.max_strain_sweeps,
+                           P.pass_timing >= 1 ? &strain_marks : nullptr));
     launch_final_velocity(h->st, S);
     launch_final_bodies(h->st, S, P.dt);  // updateFinalForRG
     HIPCHK(h, hipGetLastError());
@@ -1191,6 +1233,36 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   }
   R.ms_total = elapsed(t0, t_end);
   R.ms_strain = elapsed(t_strain0, t_end);
+  for (size_t i = 0; i + 1 < strain_marks.size(); i += 2)
+    R.ms_strain_kernel += elapsed(strain_marks[i], strain_marks[i + 1]);
+  for (hipEvent_t e : strain_marks) (void)hipEventDestroy(e);
+  R.host_syncs = h->syncs;
+  R.mx_screened = h->h_counters[CNT_MX_SCREENED];
+  R.mx_rechecked = h->h_counters[CNT_MX_RECHECK];
+  {
+    float ed, ep;
+    std::memcpy(&ed, &h->h_counters[CNT_MX_EDIST], sizeof(float));
+    std::memcpy(&ep, &h->h_counters[CNT_MX_EPARAM], sizeof(float));
+    float er;
+    std::memcpy(&er, &h->h_counters[CNT_MX_RATIO], sizeof(float));
+    R.mx_err_dist = ed;
+    R.mx_err_param = ep;
+    R.mx_bound_ratio = er;
+  }
+  if (P.learned) {
+    const int* c = h->h_counters;
+    cs_learned_stats& ls = h->lstats;
+    ls.scored += c[CNT_LN_SCORED];
+    ls.exists += c[CNT_LN_EXISTS];
+    ls.predicted_exists += c[CNT_LN_PRED];
+    ls.agree += c[CNT_LN_AGREE];
+    ls.newton_roots += c[CNT_LN_ROOTS];
+    ls.newton_iters += c[CNT_LN_ITERS];
+    ls.newton_iters_mid += c[CNT_LN_ITERS_MID];
+    float em;
+    std::memcpy(&em, &c[CNT_LN_EMAX], sizeof(float));
+    ls.max_root_err = std::max(ls.max_root_err, (double)em);
+  }
   if (report) *report = R;
   return status;
 }
@@ -1431,6 +1503,137 @@ int cs_zone_labels_noalloc(cs_solver* h, int32_t* out) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, d, sizeof(int32_t) * h->S.nv, hipMemcpyDeviceToHost, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
+  return CS_OK;
+}
+
+// ---- learned time-of-impact proposer ----
+int cs_learned_set_weights(cs_solver* h, const float* w) {
+  if (!h || !w) return CS_ERR_ARG;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, learned_set_weights(h->L, h->st, w));
+  return CS_OK;
+}
+
+int cs_learned_get_weights(cs_solver* h, float* w) {
+  if (!h || !w) return CS_ERR_ARG;
+  if (!h->L.ready) return fail(h, CS_ERR_ARG, "no weights have been set");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, learned_get_weights(h->L, h->st, w));
+  return CS_OK;
+}
+
+int cs_learned_init(cs_solver* h, uint64_t seed) {
+  if (!h) return CS_ERR_ARG;
+  std::vector<float> w(CS_MLP_PARAMS);
+  learned_init_host(seed, w.data());
+  return cs_learned_set_weights(h, w.data());
+}
+
+int cs_learned_infer(cs_solver* h, const float* x, float* out, int64_t n) {
+  if (!h || n < 0 || (n > 0 && (!x || !out))) return CS_ERR_ARG;
+  if (!h->L.ready) return fail(h, CS_ERR_ARG, "no weights have been set");
+  if (n == 0) return CS_OK;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  float *dx = nullptr, *dout = nullptr;
+  hipError_t e = dalloc(&dx, (size_t)n * 24);
+  if (e == hipSuccess) e = dalloc(&dout, (size_t)n * 2);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dx, x, sizeof(float) * 24 * (size_t)n, hipMemcpyHostToDevice, h->st);
+  if (e == hipSuccess) e = learned_infer(h->L, h->st, dx, n, dout);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, dout, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  dfree(dx);
+  dfree(dout);
+  if (e != hipSuccess) return fail(h, CS_ERR_HIP, std::string("learned inference: ") + hipGetErrorString(e));
+  return CS_OK;
+}
+
+int cs_learned_grad(cs_solver* h, const float* x, const float* exists, const float* toi,
+                    int64_t n, float* grad, float* loss) {
+  if (!h || !x || !exists || !toi || !grad || n <= 0 || n > (1 << 24)) return CS_ERR_ARG;
+  if (!h->L.ready) return fail(h, CS_ERR_ARG, "no weights have been set");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  float *dx = nullptr, *de = nullptr, *dt = nullptr, *dl = nullptr;
+  hipError_t e = dalloc(&dx, (size_t)n * 24);
+  if (e == hipSuccess) e = dalloc(&de, (size_t)n);
+  if (e == hipSuccess) e = dalloc(&dt, (size_t)n);
+  if (e == hipSuccess) e = dalloc(&dl, 1);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dx, x, sizeof(float) * 24 * (size_t)n, hipMemcpyHostToDevice, h->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(de, exists, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, h->st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dt, toi, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, h->st);
+  if (e == hipSuccess) e = learned_grad(h->L, h->st, dx, de, dt, nullptr, (int)n, dl);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(grad, h->L.grad, sizeof(float) * CS_MLP_PARAMS, hipMemcpyDeviceToHost, h->st);
+  float l = 0.f;
+  if (e == hipSuccess) e = hipMemcpyAsync(&l, dl, sizeof(float), hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  dfree(dx);
+  dfree(de);
+  dfree(dt);
+  dfree(dl);
+  if (e != hipSuccess) return fail(h, CS_ERR_HIP, std::string("learned gradient: ") + hipGetErrorString(e));
+  if (loss) *loss = l;
+  return CS_OK;
+}
+
+int cs_learned_collect(cs_solver* h, int64_t capacity, int32_t stride) {
+  if (!h || capacity < 0 || capacity > INT32_MAX / 2 || stride < 1) return CS_ERR_ARG;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, learned_alloc(h->L));
+  if (capacity == 0) {
+    h->L.ds_cap = 0;
+    dfree(h->L.dsx);
+    dfree(h->L.dse);
+    dfree(h->L.dst);
+    h->L.ds_alloc = 0;
+  } else {
+    HIPCHK(h, learned_ensure_dataset(h->L, capacity));
+    h->L.ds_cap = capacity;
+  }
+  h->L.ds_stride = stride;
+  h->L.ds_salt = 0;
+  HIPCHK(h, hipMemset(h->L.ds_count, 0, sizeof(unsigned long long)));
+  return CS_OK;
+}
+
+int64_t cs_learned_dataset_size(cs_solver* h) {
+  if (!h) return -1;
+  if (hipSetDevice(h->P.device) != hipSuccess) return -1;
+  return learned_dataset_size(h->L, h->st);
+}
+
+int cs_learned_dataset(cs_solver* h, float* x, float* exists, float* toi, int64_t cap) {
+  if (!h || cap < 0) return CS_ERR_ARG;
+  HIPCHK(h, hipSetDevice(h->P.device));
+  const int64_t n = std::min<int64_t>(learned_dataset_size(h->L, h->st), cap);
+  if (n < 0) return fail(h, CS_ERR_HIP, "dataset size");
+  if (n == 0) return CS_OK;
+  if (!x || !exists || !toi) return CS_ERR_ARG;
+  HIPCHK(h, hipMemcpy(x, h->L.dsx, sizeof(float) * 24 * (size_t)n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(exists, h->L.dse, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(toi, h->L.dst, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+  return CS_OK;
+}
+
+int cs_learned_train(cs_solver* h, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                     float* loss) {
+  if (!h || epochs < 0 || batch < 1 || !(lr > 0)) return CS_ERR_ARG;
+  if (!h->L.ready) return fail(h, CS_ERR_ARG, "no weights have been set");
+  HIPCHK(h, hipSetDevice(h->P.device));
+  if (learned_dataset_size(h->L, h->st) <= 0) return fail(h, CS_ERR_ARG, "the dataset is empty");
+  HIPCHK(h, learned_train(h->L, h->st, epochs, batch, lr, seed, loss));
+  return CS_OK;
+}
+
+int cs_learned_get_stats(cs_solver* h, cs_learned_stats* out) {
+  if (!h || !out) return CS_ERR_ARG;
+  *out = h->lstats;
+  h->lstats = cs_learned_stats{};
   return CS_OK;
 }
 

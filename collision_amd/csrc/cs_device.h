@@ -326,6 +326,41 @@ __device__ inline double cs_det_pow_third(double x) {
   return cs_dd_round(cs_dd_add(c, cg));
 }
 
+// Fast approximations (a few ulps) of the same functions, for the certified
+// screen of the coplanarity cubic: Cody-Waite reduction by pi/2 and Taylor
+// polynomials for cos; pi/2 - asin and the half-angle identities with the
+// Maclaurin series for acos; the exponent-split Newton seed for the cube root.
+__device__ inline double cs_fast_cos(double x) {
+  const double k = floor(x * CS_TWO_OVER_PI + 0.5);
+  double r = fma(-k, CS_PIO2_1, x);
+  r = fma(-k, CS_PIO2_2, r);
+  const double r2 = r * r;
+  const int q = (int)((long long)k & 3);
+  double s = -7.647163731819816e-13, c = 4.779477332387385e-14;
+  s = s * r2 + 1.6059043836821613e-10;
+  s = s * r2 - 2.505210838544172e-08;
+  s = s * r2 + 2.7557319223985893e-06;
+  s = s * r2 - 0.0001984126984126984;
+  s = s * r2 + 0.008333333333333333;
+  s = s * r2 - 0.16666666666666666;
+  s = (s * r2 + 1.0) * r;
+  c = c * r2 - 1.1470745597729725e-11;
+  c = c * r2 + 2.08767569878681e-09;
+  c = c * r2 - 2.755731922398589e-07;
+  c = c * r2 + 2.48015873015873e-05;
+  c = c * r2 - 0.001388888888888889;
+  c = c * r2 + 0.041666666666666664;
+  c = c * r2 - 0.5;
+  c = c * r2 + 1.0;
+  return q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+__device__ inline double cs_fast_acos(double x) {
+  if (x <= 0.5 && x >= -0.5) return (CS_PIO2_1 - cs_asin_seed(x)) + CS_PIO2_2;
+  const double t = (x > 0.0 ? (1.0 - x) : (1.0 + x)) * 0.5;
+  const double a = 2.0 * cs_asin_seed(sqrt(t));
+  return x > 0.0 ? a : (CS_PI_1 - a) + CS_PI_2;
+}
+
 // Round a double to float toward -inf / +inf (conservative FP32 boxes).
 __device__ inline float f32_down(double v) { return __double2float_rd(v); }
 __device__ inline float f32_up(double v) { return __double2float_ru(v); }

@@ -52,7 +52,20 @@ enum {
   CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
   CNT_PCAND64 = 32,      // 32, 33: static-box overlaps found by the swept-box traversal
   CNT_PPAIRS = 34,       // pairs of that traversal whose static boxes overlap
-  CNT_INTS = 40
+  CNT_MX_SCREENED = 35,  // mixed precision: sub-tests the FP32 screen rejected (whole step)
+  CNT_MX_RECHECK = 36,   // sub-tests it passed on to FP64
+  CNT_MX_EDIST = 37,     // largest FP32 distance error / thickness (float bits)
+  CNT_MX_EPARAM = 38,    // largest FP32 barycentric / segment parameter error (float bits)
+  CNT_MX_RATIO = 39,     // largest FP32 error / its certified bound (float bits)
+  CNT_LN_SCORED = 40,    // learned form (whole step): CCD sub-tests scored by the network
+  CNT_LN_EXISTS = 41,    // of those, isCoplanar true (exact cubic)
+  CNT_LN_PRED = 42,      // predicted to have a root (logit > 0)
+  CNT_LN_AGREE = 43,     // prediction == exact outcome
+  CNT_LN_ROOTS = 44,     // roots located by the seeded Newton solve
+  CNT_LN_ITERS = 45,     // its iterations, seeded at the network's time of impact
+  CNT_LN_ITERS_MID = 46, // iterations of the same solve seeded at dt / 2
+  CNT_LN_EMAX = 47,      // largest |Newton root - closed-form root| / dt (float bits)
+  CNT_INTS = 48
 };
 
 struct NarrowParams {
@@ -60,9 +73,25 @@ struct NarrowParams {
   const int32_t* vbody;  // body of a movable vertex, -1 otherwise (nullptr: no bodies)
   const double* bmass;   // total_mass of each body
   int ccd;        // 0 proximity, 1 CCD
+  int mixed;      // FP32 screen of proximity sub-tests
+  int mixed_stats;
   int zone_mode;  // merge impact zones (s_detImpZone)
   int log_pairs;
   int pass_id;
+  // learned form (CCD passes): the scoring kernel's per-sub-test seeds
+  // (16 per pair, by pair position before reordering; sign bit set = root
+  // predicted) and the reordering of the pairs (nullptr: none)
+  const float* lseed;
+  const uint32_t* perm;
+  // dataset collection (ds_cap == 0: off): keep a CCD sub-test when a hash
+  // of (pair, sub-test, salt) is divisible by ds_stride
+  float* dsx;
+  float* dse;
+  float* dst;
+  unsigned long long* ds_count;
+  long long ds_cap;
+  int ds_stride;
+  uint32_t ds_salt;
 };
 
 // One slot of the strain-limit schedule (one non-rigid element), 256 B.
@@ -194,6 +223,45 @@ struct DevState {
   int* counters = nullptr;
 };
 
+// The time-of-impact proposer of a handle: FP32 master weights, Adam state,
+// BF16 matrix-core copies, the scoring and sorting buffers of the learned
+// form and the collected dataset.
+struct LearnedState {
+  float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  uint16_t* wb = nullptr;
+  bool ready = false;  // weights set
+  int64_t adam_t = 0;
+  // learned form: seeds [16 * cap], sort keys / positions and their partners
+  float* seeds = nullptr;
+  uint32_t *keys = nullptr, *idx = nullptr, *tkeys = nullptr, *tidx = nullptr;
+  uint32_t *hist = nullptr, *part = nullptr;
+  int score_cap = 0;
+  // dataset: x [cap][24], exists [cap], toi [cap]; ds_count counts every
+  // selected sub-test (those beyond ds_cap are dropped)
+  float *dsx = nullptr, *dse = nullptr, *dst = nullptr;
+  unsigned long long* ds_count = nullptr;
+  int64_t ds_alloc = 0, ds_cap = 0;
+  int32_t ds_stride = 1;
+  uint32_t ds_salt = 0;
+  // training partials
+  float *gpart = nullptr, *lpart = nullptr;
+  int train_tiles = 0;
+};
+
+hipError_t learned_alloc(LearnedState& L);
+void learned_free(LearnedState& L);
+hipError_t learned_set_weights(LearnedState& L, hipStream_t st, const float* host_w);
+hipError_t learned_get_weights(LearnedState& L, hipStream_t st, float* host_w);
+void learned_init_host(uint64_t seed, float* w);
+hipError_t learned_infer(LearnedState& L, hipStream_t st, const float* dx, int64_t n, float* dout);
+hipError_t learned_grad(LearnedState& L, hipStream_t st, const float* dx, const float* de,
+                        const float* dtoi, const uint32_t* idx, int n, float* dloss);
+hipError_t learned_adam_step(LearnedState& L, hipStream_t st, float lr);
+hipError_t learned_ensure_dataset(LearnedState& L, int64_t capacity);
+int64_t learned_dataset_size(LearnedState& L, hipStream_t st);
+hipError_t learned_train(LearnedState& L, hipStream_t st, int epochs, int batch, float lr,
+                         uint64_t seed, float* host_loss);
+
 // sort and scan
 size_t radix_hist_words(int n);
 size_t scan_partial_words(int n);
@@ -236,6 +304,12 @@ void launch_carry_pairs_from(hipStream_t st, DevState& S, const int2* src, int n
 // narrow phase
 void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int2* pairs,
                    int npairs, const int* npairs_dev, int cap);
+// learned form: score a pair stream's CCD sub-tests and sort the pairs by
+// decreasing score (*perm: positions in test order; the narrow phase reads
+// L.seeds through it)
+hipError_t launch_learned_score(hipStream_t st, DevState& S, LearnedState& L, const int2* pairs,
+                                int npairs, const int* npairs_dev, int cap, double dt, double h,
+                                const uint32_t** perm);
 void launch_hp_unions(hipStream_t st, DevState& S, int n, int ccd);
 void launch_hp_log(hipStream_t st, DevState& S, int n, int pass_id);
 // y = f(x) with the step's elementary functions (0 acos, 1 cos, 2 sin, 3 pow(x, 1/3))
@@ -259,7 +333,10 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nl
 void launch_domain_boundary(hipStream_t st, DevState& S, const double lo[3], const double hi[3],
                             double dt, double mu, double thickness);
 void launch_final_position(hipStream_t st, DevState& S, double dt);
-hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps);
+// marks != nullptr: a pair of events around every k_strain_pipe launch is
+// appended (the caller reads and destroys them)
+hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps,
+                        std::vector<hipEvent_t>* marks);
 // cs_schedule.hip (host)
 void build_strain_schedule(int nv, int ne, const int4* ev, const uint8_t* ef,
                            std::vector<int>& level, int& n_levels, int& delta);

@@ -18,11 +18,15 @@
 // hit onward are merged (the reference's cumulative-status rule,
 // dcollid3d.cpp:222-229, 300-302, 319-321) by a lock-free union-find whose
 // roots are the smallest vertex id of each zone.
+#include <algorithm>
+
 #include "cs_internal.h"
+#include "cs_mlp.h"
 
 namespace cs {
 
 constexpr int NARROW_THREADS = 128;
+constexpr int SUB_GROUP = 16;  // lanes per pair (sub-tests 0..14)
 
 struct Sub {
   double xo[4][3];
@@ -331,6 +335,89 @@ __device__ bool d_edge_to_edge(const Sub& S, const double c[4][3], double h, con
   return true;
 }
 
+// Certified screen of isCoplanar: true when isCoplanar (below) certainly
+// returns false.  The cubic's coefficients, Q, R and every branch decision are
+// computed exactly as below; the roots are then taken with the fast
+// acos / cos / cube root (a few ulps; the cube root also stands for
+// pow(x, 1.0/3.0), which differs from it by at most 1.85e-17 |ln x|
+// relative).  Each approximate root is within margin = 1e-13 (|scale| + dt)
+// of the correctly rounded one, about a hundred times the worst error of
+// those approximations; a root counts as certainly invalid only when it lies
+// beyond the validity window (2 MACH_EPS, dt + MACH_EPS] by more than the
+// margin, and the Cardano branch's two thresholds on |A| and |A - B| must be
+// cleared by the margin as well.  Anything else -- NaN included -- is not
+// certain and runs the exact path.
+__device__ bool d_coplanar_certainly_false(const Sub& S, double dt) {
+  double v[4][3], x[4][3];
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[i][k] = S.v[i][k] - S.v[0][k];
+      x[i][k] = S.xo[i][k] - S.xo[0][k];
+    }
+  double a, b, c, d, vv[3], vx[3], xx[3];
+  vv[0] = v[1][1] * v[2][2] - v[1][2] * v[2][1];
+  vv[1] = v[1][0] * v[2][2] - v[1][2] * v[2][0];
+  vv[2] = v[1][0] * v[2][1] - v[1][1] * v[2][0];
+  vx[0] = v[1][1] * x[2][2] - v[1][2] * x[2][1] - v[2][1] * x[1][2] + v[2][2] * x[1][1];
+  vx[1] = v[1][0] * x[2][2] - v[1][2] * x[2][0] - v[2][0] * x[1][2] + v[2][2] * x[1][0];
+  vx[2] = v[1][0] * x[2][1] - v[1][1] * x[2][0] - v[2][0] * x[1][1] + v[2][1] * x[1][0];
+  xx[0] = x[1][1] * x[2][2] - x[1][2] * x[2][1];
+  xx[1] = x[1][0] * x[2][2] - x[1][2] * x[2][0];
+  xx[2] = x[1][0] * x[2][1] - x[1][1] * x[2][0];
+  a = v[3][0] * vv[0] - v[3][1] * vv[1] + v[3][2] * vv[2];
+  b = x[3][0] * vv[0] - x[3][1] * vv[1] + x[3][2] * vv[2] + v[3][0] * vx[0] - v[3][1] * vx[1] +
+      v[3][2] * vx[2];
+  c = x[3][0] * vx[0] - x[3][1] * vx[1] + x[3][2] * vx[2] + v[3][0] * xx[0] - v[3][1] * xx[1] +
+      v[3][2] * xx[2];
+  d = x[3][0] * xx[0] - x[3][1] * xx[1] + x[3][2] * xx[2];
+  if (!(fabs(a) > CS_MACH_EPS)) return false;  // the quadratic / linear branch is exact and cheap
+  b /= a;
+  c /= a;
+  d /= a;
+  a = b;
+  b = c;
+  c = d;
+  const double Q = (a * a - 3 * b) / 9;
+  const double R = (2 * a * a * a - 9 * a * b + 27 * c) / 54;
+  const double Q3 = Q * Q * Q, R2 = R * R;
+  double r[3];
+  int nr = 0;
+  double scale = fabs(a / 3);
+  if (R2 < Q3) {
+    const double Qsqrt = sqrt(Q);
+    const double theta = cs_fast_acos(R / sqrt(Q3));
+    r[0] = -2 * Qsqrt * cs_fast_cos(theta / 3) - a / 3;
+    r[1] = -2 * Qsqrt * cs_fast_cos((theta + 2 * CS_PI) / 3) - a / 3;
+    r[2] = -2 * Qsqrt * cs_fast_cos((theta - 2 * CS_PI) / 3) - a / 3;
+    nr = 3;
+    scale += 2 * Qsqrt;
+  } else {
+    const double sgn = (R > 0) ? 1.0 : -1.0;
+    const double A = -sgn * cs_cbrt_seed(fabs(R) + sqrt(R2 - Q3));
+    const double mA = 1e-13 * fabs(A);
+    if (!(fabs(fabs(A) - CS_ROUND_EPS) > mA)) return false;  // near the B = 0 threshold
+    const double B = (fabs(A) < CS_ROUND_EPS) ? 0.0 : Q / A;
+    scale += fabs(A) + fabs(B);
+    r[0] = (A + B) - a / 3.0;
+    nr = 1;
+    if (!(fabs(fabs(A - B) - CS_ROUND_EPS) > 1e-13 * (fabs(A) + fabs(B))))
+      return false;  // near the double-root threshold
+    if (fabs(A - B) < CS_ROUND_EPS) {
+      r[1] = -0.5 * (A + B) - a / 3.0;
+      nr = 2;
+    }
+  }
+  const double margin = 1e-13 * (scale + dt);
+  for (int i = 0; i < nr; ++i) {
+    const double t = r[i] - CS_MACH_EPS;
+    // counts for isCoplanar iff MACH_EPS < t <= dt
+    if (!(t + margin <= CS_MACH_EPS || t - margin > dt)) return false;
+  }
+  return true;
+}
+
 // isCoplanar (dcollid3d.cpp:371-482)
 __device__ bool d_is_coplanar(const Sub& S, double dt, double roots[4]) {
   double v[4][3], x[4][3];
@@ -412,7 +499,10 @@ __device__ bool d_is_coplanar(const Sub& S, double dt, double roots[4]) {
   return roots[0] > CS_MACH_EPS || roots[1] > CS_MACH_EPS || roots[2] > CS_MACH_EPS;
 }
 
-__device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contrib& C, int* err) {
+// roots / *coplanar: isCoplanar's outcome for a CCD sub-test (roots -1 and
+// false when the certified screen decided it)
+__device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contrib& C, int* err,
+                          double roots[4], bool* coplanar) {
   double c[4][3];
   if (!P.ccd) {
 #pragma unroll
@@ -422,8 +512,12 @@ __device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contr
     return is_pt ? d_point_to_tri(S, c, P.thickness, P, C, err)
                  : d_edge_to_edge(S, c, P.thickness, P, C, err);
   }
-  double roots[4];
+  roots[0] = roots[1] = roots[2] = -1.0;
+  roots[3] = P.dt;
+  *coplanar = false;
+  if (P.mixed && d_coplanar_certainly_false(S, P.dt)) return false;
   if (!d_is_coplanar(S, P.dt, roots)) return false;
+  *coplanar = true;
   for (int r = 0; r < 4; ++r) {
     if (roots[r] < 0) continue;
 #pragma unroll
@@ -435,6 +529,163 @@ __device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contr
     if (hit) return true;
   }
   return false;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 screen of a static proximity sub-test.
+//
+// The points are taken relative to a local origin (point 2 for PointToTri,
+// point 0 for EdgeToEdge, as the FP64 formulas do) in FP64 and rounded once
+// to FP32, so every input carries a relative error of at most U = 2^-24
+// (plus 2^-53).  Each decision quantity then gets a forward-error bound from
+// the magnitudes of its operands (m_i = |x_i|):
+//   dot product of two inputs      <= 5.1 U |a| |b|   (input rounding + 3 terms)
+//   cross-product component        <= 4.1 U |a| |b|   (vector: <= 7.1 U |a| |b|)
+//   det = d11 d22 - d12^2          <= 24 U m1^2 m2^2
+//   N0, N1 (barycentric numerators)<= 24 U m1 m2 m3 max(m1, m2)
+//   x43 . n, |n|                   <= 11.1 U m1 m2 m3,  9.1 U m1 m2
+// and the code uses bounds with a safety factor of two to three above these.
+// The screen reports a certain reject only when a quantity lies outside the
+// FP64 test's acceptance region by more than its bound (and by a relative
+// 1e-9 for the FP64 test's own rounding): then the FP64 test rejects too.
+// Every other sub-test runs the FP64 test unchanged, so results are those of
+// the FP64 path bit for bit.  Where the FP64 test would flag a degenerate
+// normal (the reference's clean_up(ERROR)), the screen never rejects: it
+// requires a certified non-degenerate triangle, and an edge pair's flag is
+// raised only after its distance test has passed.
+constexpr float MX_U = 5.9604645e-8f;  // 2^-24
+
+struct MixedErr {
+  float dist, param;  // FP32 - FP64 errors, for the statistics
+  float ratio;        // largest |FP32 - FP64| / certified bound of a screened quantity
+};
+__device__ inline void mx_ratio(MixedErr* me, double err, float bound) {
+  if (bound > 0.0f) me->ratio = fmaxf(me->ratio, (float)(err / (double)bound));
+}
+
+__device__ inline float f3dot(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ inline float f3mag(const float* a) { return sqrtf(f3dot(a, a)); }
+
+// true: the FP64 PointToTri with thickness h and barycentric tolerance eps
+// certainly rejects
+__device__ bool screen_pt(const Sub& S, double h, double eps, bool stats, MixedErr* me) {
+  float x13[3], x23[3], x43[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x13[k] = (float)(S.xo[0][k] - S.xo[2][k]);
+    x23[k] = (float)(S.xo[1][k] - S.xo[2][k]);
+    x43[k] = (float)(S.xo[3][k] - S.xo[2][k]);
+  }
+  const float m1 = f3mag(x13) * 1.000001f, m2 = f3mag(x23) * 1.000001f, m3 = f3mag(x43) * 1.000001f;
+  const float d11 = f3dot(x13, x13), d22 = f3dot(x23, x23), d12 = f3dot(x13, x23);
+  const float d13 = f3dot(x13, x43), d23 = f3dot(x23, x43);
+  const float det = d11 * d22 - d12 * d12;
+  const float e_det = 64.0f * MX_U * (m1 * m1) * (m2 * m2);
+  if (!(det > 2.0f * e_det) || !(det > 1e-30f)) return false;  // not certainly well shaped
+  const float n0 = d13 * d22 - d23 * d12, n1 = d11 * d23 - d12 * d13;
+  const float mm = fmaxf(m1, m2);
+  const float e_n = 64.0f * MX_U * m1 * m2 * m3 * mm;
+  const float w0 = n0 / det, w1 = n1 / det, w2 = 1.0f - w0 - w1;
+  const float den = det - e_det;
+  const float ew0 = (e_n + fabsf(w0) * e_det) / den + 4.0f * MX_U * fabsf(w0);
+  const float ew1 = (e_n + fabsf(w1) * e_det) / den + 4.0f * MX_U * fabsf(w1);
+  const float ew2 = ew0 + ew1 + 4.0f * MX_U * (1.0f + fabsf(w0) + fabsf(w1));
+  float n[3] = {x13[1] * x23[2] - x13[2] * x23[1], x13[2] * x23[0] - x13[0] * x23[2],
+                x13[0] * x23[1] - x13[1] * x23[0]};
+  const float nn = f3mag(n);
+  const float e_nn = 28.0f * MX_U * m1 * m2;
+  if (!(nn > 2.0f * e_nn)) return false;
+  const float num = fabsf(f3dot(x43, n));
+  const float dist = num / nn;
+  const float e_dist = (32.0f * MX_U * m1 * m2 * m3 + dist * e_nn) / (nn - e_nn) + 4.0f * MX_U * dist;
+  if (stats) {
+    // the FP64 quantities of the same sub-test (PointToTri's formulas)
+    double a[3], b[3], c[3], nd[3];
+    sub3(S.xo[0], S.xo[2], a);
+    sub3(S.xo[1], S.xo[2], b);
+    sub3(S.xo[3], S.xo[2], c);
+    const double D = dot3(a, a) * dot3(b, b) - dot3(a, b) * dot3(a, b);
+    const double W0 = (dot3(a, c) * dot3(b, b) - dot3(b, c) * dot3(a, b)) / D;
+    const double W1 = (dot3(a, a) * dot3(b, c) - dot3(a, b) * dot3(a, c)) / D;
+    cross3(a, b, nd);
+    const double dd = fabs(dot3(c, nd)) / mag3(nd);
+    me->dist = fmaxf(me->dist, (float)(fabs((double)dist - dd) / h));
+    me->param = fmaxf(me->param, (float)fmax(fabs((double)w0 - W0), fabs((double)w1 - W1)));
+    mx_ratio(me, fabs((double)dist - dd), e_dist);
+    mx_ratio(me, fabs((double)w0 - W0), ew0);
+    mx_ratio(me, fabs((double)w1 - W1), ew1);
+    mx_ratio(me, fabs((double)w2 - (1.0 - W0 - W1)), ew2);
+  }
+  const float slack = 1e-9f;
+  if ((double)(dist - e_dist) > h * (1.0 + 1e-9)) return true;
+  const float w[3] = {w0, w1, w2}, ew[3] = {ew0, ew1, ew2};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if ((double)(w[i] - ew[i]) > 1.0 + eps + slack) return true;
+    if ((double)(w[i] + ew[i]) < -eps - slack) return true;
+  }
+  return false;
+}
+
+// true: the FP64 EdgeToEdge with thickness h certainly rejects.  The clamped
+// segment distance is at least the gap between the segments' boxes and at
+// least the distance between the two lines.
+__device__ bool screen_ee(const Sub& S, double h, bool stats, MixedErr* me) {
+  float p[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[q][k] = (float)(S.xo[q][k] - S.xo[0][k]);
+  float gap = 0.0f, ext = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float alo = fminf(p[0][k], p[1][k]), ahi = fmaxf(p[0][k], p[1][k]);
+    const float blo = fminf(p[2][k], p[3][k]), bhi = fmaxf(p[2][k], p[3][k]);
+    gap = fmaxf(gap, fmaxf(blo - ahi, alo - bhi));
+    ext = fmaxf(ext, fmaxf(fmaxf(fabsf(alo), fabsf(ahi)), fmaxf(fabsf(blo), fabsf(bhi))));
+  }
+  // a box gap: two subtractions of inputs of relative error U
+  const float e_gap = 4.0f * MX_U * ext;
+  if (stats) {  // the same gap from the FP64 differences
+    double G = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double a0 = 0.0, a1 = S.xo[1][k] - S.xo[0][k];
+      const double b0 = S.xo[2][k] - S.xo[0][k], b1 = S.xo[3][k] - S.xo[0][k];
+      G = fmax(G, fmax(fmin(b0, b1) - fmax(a0, a1), fmin(a0, a1) - fmax(b0, b1)));
+    }
+    me->dist = fmaxf(me->dist, (float)(fabs((double)gap - G) / h));
+    mx_ratio(me, fabs((double)gap - G), e_gap);
+  }
+  if ((double)(gap - e_gap) > h * (1.0 + 1e-9)) return true;
+  float x21[3], x43[3], x31[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x21[k] = p[1][k];
+    x43[k] = p[3][k] - p[2][k];
+    x31[k] = p[2][k];
+  }
+  const float m21 = f3mag(x21) * 1.000001f, m43 = f3mag(x43) * 1.000001f, m31 = f3mag(x31) * 1.000001f;
+  float n[3] = {x21[1] * x43[2] - x21[2] * x43[1], x21[2] * x43[0] - x21[0] * x43[2],
+                x21[0] * x43[1] - x21[1] * x43[0]};
+  const float nn = f3mag(n);
+  // x43 = p3 - p2 carries one more subtraction: its error is at most 3 U m43 + 2 U ext
+  const float e_nn = 40.0f * MX_U * (m21 * (m43 + ext));
+  if (!(nn > 2.0f * e_nn)) return false;
+  const float num = fabsf(f3dot(x31, n));
+  const float lined = num / nn;
+  const float e_num = 48.0f * MX_U * m31 * m21 * (m43 + ext);
+  const float e_line = (e_num + lined * e_nn) / (nn - e_nn) + 4.0f * MX_U * lined;
+  if (stats) {
+    double a[3], b[3], c[3], nd[3];
+    sub3(S.xo[1], S.xo[0], a);
+    sub3(S.xo[3], S.xo[2], b);
+    sub3(S.xo[2], S.xo[0], c);
+    cross3(a, b, nd);
+    const double L = fabs(dot3(c, nd)) / mag3(nd);
+    me->dist = fmaxf(me->dist, (float)(fabs((double)lined - L) / h));
+    mx_ratio(me, fabs((double)lined - L), e_line);
+  }
+  return (double)(lined - e_line) > h * (1.0 + 1e-9);
 }
 
 // local point indices of sub-test s.  kind: 0 tri-tri (A 0..2, B 3..5),
@@ -525,6 +776,95 @@ __device__ inline void zone_merge4(const ZoneList& Z, const int32_t* vbody, cons
   }
 }
 
+// vertices of an element pair (canonical orientation: A = hi, B = lo) and
+// its kind: 0 tri-tri (15 sub-tests), 1 tri-bond (5), 2 bond-bond (1)
+__device__ inline void pair_points(int2 pr, const int4* __restrict__ ev, int* A, int* B, int* kind,
+                                   int* nsub, int g[6]) {
+  *A = pr.y;
+  *B = pr.x;
+  const int4 ta = ev[*A], tb = ev[*B];
+  const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
+  if (tri_a && tri_b) {
+    *kind = 0; *nsub = 15;
+    g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
+  } else if (tri_a || tri_b) {
+    const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
+    *kind = 1; *nsub = 5;
+    g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
+  } else {
+    *kind = 2; *nsub = 1;
+    g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
+  }
+}
+
+__device__ inline uint32_t subtest_hash(int B, int A, int s, uint32_t salt) {
+  uint64_t z = ((uint64_t)(uint32_t)B << 32 | (uint32_t)A) ^ ((uint64_t)s << 58) ^
+               ((uint64_t)salt * 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return (uint32_t)((z ^ (z >> 31)) >> 32);
+}
+
+// first root isCoplanar reports valid (the roots are sorted; -1 = invalid)
+__device__ inline double first_valid_root(const double roots[4]) {
+  for (int r = 0; r < 3; ++r)
+    if (roots[r] >= 0) return roots[r];
+  return -1.0;
+}
+
+// dataset sample of a CCD sub-test: features, root-exists label (isCoplanar)
+// and the first valid root / dt
+__device__ void collect_subtest(const Sub& S, const NarrowParams& P, int B, int A, int s,
+                                bool coplanar, const double roots[4]) {
+  if (subtest_hash(B, A, s, P.ds_salt) % (uint32_t)P.ds_stride != 0) return;
+  const unsigned long long slot = atomicAdd(P.ds_count, 1ull);
+  if ((long long)slot >= P.ds_cap) return;
+  float f[MF_IN];
+  subtest_features(S.xo, S.v, P.dt, P.thickness > 0 ? P.thickness : 1.0, f);
+  for (int k = 0; k < MF_IN; ++k) P.dsx[slot * MF_IN + k] = f[k];
+  const double r = coplanar ? first_valid_root(roots) : -1.0;
+  P.dse[slot] = coplanar ? 1.f : 0.f;
+  P.dst[slot] = (coplanar && r >= 0) ? (float)(r / P.dt) : 0.f;
+}
+
+// Learned-form statistics of a CCD sub-test (all lanes of the wavefront
+// call it): prediction vs isCoplanar, and for a sub-test with roots the
+// bracketed Newton solve from the network's seed and from dt / 2, compared
+// with the nearest valid closed-form root (+ MACH_EPS, which isCoplanar
+// subtracts)
+__device__ void learned_stats(const NarrowParams& P, bool active, bool coplanar,
+                              const double roots[4], int64_t pi, int s, const Sub& S,
+                              int* counters) {
+  bool pred = false;
+  if (active) {
+    const float sv = P.lseed[pi * SUB_GROUP + s];
+    pred = signbit(sv);
+    if (coplanar) {
+      double a, b, c, d, rn = 0.0, rm = 0.0;
+      cubic_coeffs(S.xo, S.v, &a, &b, &c, &d);
+      const int it = newton_first_root(a, b, c, d, P.dt, (double)fabsf(sv) * P.dt, &rn);
+      const int im = newton_first_root(a, b, c, d, P.dt, 0.5 * P.dt, &rm);
+      if (it >= 0) {
+        double e = 1e300;
+        for (int r = 0; r < 3; ++r)
+          if (roots[r] >= 0) e = fmin(e, fabs(rn - (roots[r] + CS_MACH_EPS)));
+        atomicAdd(&counters[CNT_LN_ROOTS], 1);
+        atomicAdd(&counters[CNT_LN_ITERS], it);
+        atomicAdd(&counters[CNT_LN_ITERS_MID], im >= 0 ? im : 0);
+        if (e < 1e300) atomicMax(&counters[CNT_LN_EMAX], __float_as_int((float)(e / P.dt)));
+      }
+    }
+  }
+  const uint64_t b_sc = __ballot(active), b_ex = __ballot(active && coplanar);
+  const uint64_t b_pr = __ballot(active && pred), b_ag = __ballot(active && pred == coplanar);
+  if ((threadIdx.x & 63) == 0 && b_sc) {
+    atomicAdd(&counters[CNT_LN_SCORED], __popcll(b_sc));
+    atomicAdd(&counters[CNT_LN_EXISTS], __popcll(b_ex));
+    atomicAdd(&counters[CNT_LN_PRED], __popcll(b_pr));
+    atomicAdd(&counters[CNT_LN_AGREE], __popcll(b_ag));
+  }
+}
+
 // One group of 16 lanes per pair, lane s runs sub-test s (15 for two
 // triangles, 5 for a triangle and a bond, 1 for two bonds), so the dependent
 // chain of a lane is one sub-test.  The group's hit mask comes from one
@@ -532,7 +872,6 @@ __device__ inline void zone_merge4(const ZoneList& Z, const int32_t* vbody, cons
 // first hit onward performs its own zone merges (the cumulative-status
 // rule).  Hit records take one atomic per wavefront; their order does not
 // matter (they are sorted by key before use).
-constexpr int SUB_GROUP = 16;
 
 __device__ __attribute__((always_inline)) inline void narrow_lane(
     int64_t gl, const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
@@ -546,27 +885,15 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
   const int lane = threadIdx.x & 63;
   int kind = 0, nsub = 0, A = 0, B = 0;
   int g[6] = {0, 0, 0, 0, 0, 0};
+  int64_t pi = i;  // position of the pair in the stream
   if (i < npairs) {
-    const int2 pr = pairs[i];
-    A = pr.y;
-    B = pr.x;  // canonical orientation: a = hi, b = lo
-    const int4 ta = ev[A], tb = ev[B];
-    const bool tri_a = ta.z >= 0, tri_b = tb.z >= 0;
-    if (tri_a && tri_b) {
-      kind = 0; nsub = 15;
-      g[0] = ta.x; g[1] = ta.y; g[2] = ta.z; g[3] = tb.x; g[4] = tb.y; g[5] = tb.z;
-    } else if (tri_a || tri_b) {
-      const int4 T = tri_a ? ta : tb, Bd = tri_a ? tb : ta;
-      kind = 1; nsub = 5;
-      g[0] = T.x; g[1] = T.y; g[2] = T.z; g[3] = Bd.x; g[4] = Bd.y; g[5] = Bd.y;
-    } else {
-      kind = 2; nsub = 1;
-      g[0] = ta.x; g[1] = ta.y; g[2] = tb.x; g[3] = tb.y; g[4] = tb.y; g[5] = tb.y;
-    }
+    if (P.perm) pi = P.perm[i];
+    pair_points(pairs[pi], ev, &A, &B, &kind, &nsub, g);
   }
   const bool active = s < nsub;
   int q[4] = {0, 0, 0, 0};
-  bool hit = false;
+  bool hit = false, rejected = false, coplanar = false;
+  double roots[4] = {-1.0, -1.0, -1.0, -1.0};
   Sub S;
   Contrib C;
   if (active) {
@@ -586,7 +913,26 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
       S.mv[p] = S.bd[p] >= 0;
       q[p] = vid;  // from here on: global vertex ids
     }
-    hit = d_subtest(S, is_pt, P, C, err);
+    if (P.mixed && !P.ccd) {
+      MixedErr me{0.0f, 0.0f, 0.0f};
+      rejected = is_pt ? screen_pt(S, P.thickness, P.round_eps, P.mixed_stats, &me)
+                       : screen_ee(S, P.thickness, P.mixed_stats, &me);
+      if (P.mixed_stats) {
+        atomicMax(&counters[CNT_MX_EDIST], __float_as_int(me.dist));
+        atomicMax(&counters[CNT_MX_EPARAM], __float_as_int(me.param));
+        atomicMax(&counters[CNT_MX_RATIO], __float_as_int(me.ratio));
+      }
+    }
+    if (!rejected) hit = d_subtest(S, is_pt, P, C, err, roots, &coplanar);
+    if (P.ccd && P.ds_cap > 0) collect_subtest(S, P, B, A, s, coplanar, roots);
+  }
+  if (P.ccd && P.lseed) learned_stats(P, active, coplanar, roots, pi, s, S, counters);
+  if (P.mixed && !P.ccd) {  // every lane of the wavefront takes part in the ballots
+    const uint64_t rb = __ballot(active && rejected), kb = __ballot(active && !rejected);
+    if (lane == 0 && (rb | kb)) {
+      atomicAdd(&counters[CNT_MX_SCREENED], __popcll(rb));
+      atomicAdd(&counters[CNT_MX_RECHECK], __popcll(kb));
+    }
   }
   // hit records: one atomic per wavefront
   const uint64_t hb = __ballot(hit);
@@ -633,10 +979,13 @@ __global__ void __launch_bounds__(NARROW_THREADS, 3)
              ZoneList Z, hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp,
              int hp_cap, int* __restrict__ counters) {
   const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
-  // XCD-contiguous shares of the pairs when the grid matches the work; with a
-  // grid sized from a guess the busy blocks are the first ones, and the
-  // hardware's round-robin order spreads them over all XCDs
-  const int b = npairs_dev ? (int)blockIdx.x : xcd_block(blockIdx.x, gridDim.x);
+  // XCD-contiguous shares of the pairs: the hardware deals work-groups to the
+  // 8 XCDs round-robin, so the first `busy` of them (those the pair count
+  // needs; with a count read on the device the grid may be larger) are spread
+  // evenly, and remapping within them gives each XCD one contiguous share
+  const int busy = (int)(((int64_t)n * SUB_GROUP + NARROW_THREADS - 1) / NARROW_THREADS);
+  if ((int)blockIdx.x >= busy) return;
+  const int b = xcd_block(blockIdx.x, busy);
   const int64_t gl = (int64_t)b * NARROW_THREADS + threadIdx.x;
   if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
   narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
@@ -656,6 +1005,116 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
   hipLaunchKernelGGL(k_narrow, dim3((unsigned)blocks), dim3(NARROW_THREADS), 0, st, pairs, npairs,
                      npairs_dev, cap, S.ev, S.x_old, S.av, S.fixed, P, zone_list(S), S.hits,
                      S.hit_cap, S.hp, S.hp_cap, S.counters);
+}
+
+// Learned form: every CCD sub-test of the pair stream through the network,
+// four pairs (64 sub-test slots, the 16-lane groups of the narrow phase) per
+// tile.  Each slot keeps its seed sigmoid(s) in [0, 1] with the sign bit set
+// when the logit predicts a root; each pair gets the key that sorts it by
+// decreasing best logit (pairs past the device-side count sort last, after
+// every counted pair, since the sort is stable).
+__global__ void __launch_bounds__(MLP_THREADS)
+    k_learned_score(const int2* __restrict__ pairs, int npairs, const int* __restrict__ npairs_dev,
+                    int cap, const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
+                    const dvec4* __restrict__ av, double dt, double h,
+                    const uint16_t* __restrict__ wb, const float* __restrict__ w,
+                    float* __restrict__ seeds, uint32_t* __restrict__ keys,
+                    uint32_t* __restrict__ idx) {
+  __shared__ __align__(16) uint16_t x[MT * MF_INP];
+  __shared__ __align__(16) uint16_t h1[MT * MF_H];
+  __shared__ __align__(16) uint16_t h2[MT * MF_H];
+  __shared__ float o[MT * MF_OUT];
+  const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
+  const int ntile = (npairs + 3) / 4;
+  const int t = threadIdx.x;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int i = tile * 4 + t / SUB_GROUP, s = t % SUB_GROUP;
+    bool act = false;
+    if (t < MT) {
+      float f[MF_IN];
+      if (i < n) {
+        int A, B, kind, nsub, g[6];
+        pair_points(pairs[i], ev, &A, &B, &kind, &nsub, g);
+        if (s < nsub) {
+          act = true;
+          int q[4];
+          bool is_pt;
+          subtest_points(kind, s, 1, q, &is_pt);
+          double xo[4][3], v[4][3];
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            int vid = g[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k)
+              if (q[p] == k) vid = g[k];
+            const dvec4 a = x_old[vid], b = av[vid];
+            xo[p][0] = a.x; xo[p][1] = a.y; xo[p][2] = a.z;
+            v[p][0] = b.x; v[p][1] = b.y; v[p][2] = b.z;
+          }
+          subtest_features(xo, v, dt, h, f);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < MF_INP; ++k)
+        x[t * MF_INP + k] = (act && k < MF_IN) ? f2bf(f[k]) : (uint16_t)0;
+    }
+    __syncthreads();
+    tile_forward(x, h1, h2, o, wb, w);
+    if (t < MT) {  // wavefront 0: one slot per lane
+      float z = -INFINITY;
+      if (act) {
+        z = o[t * MF_OUT];
+        const float sd = mlp_sigmoid(o[t * MF_OUT + 1]);
+        seeds[(int64_t)i * SUB_GROUP + s] = z > 0.f ? -sd : sd;
+      }
+      for (int off = SUB_GROUP / 2; off > 0; off >>= 1) z = fmaxf(z, __shfl_xor(z, off, SUB_GROUP));
+      if (s == 0 && i < npairs) {
+        uint32_t u = __float_as_uint(z);
+        u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // order-preserving bits
+        keys[i] = i < n ? ~u : 0xffffffffu;
+        idx[i] = (uint32_t)i;
+      }
+    }
+    __syncthreads();  // the tile's LDS is rewritten by the next one
+  }
+}
+
+// scores the stream and sorts it; *perm = the pair positions in test order
+hipError_t launch_learned_score(hipStream_t st, DevState& S, LearnedState& L, const int2* pairs,
+                                int npairs, const int* npairs_dev, int cap, double dt, double h,
+                                const uint32_t** perm) {
+  *perm = nullptr;
+  if (npairs <= 0) return hipSuccess;
+  if (npairs > L.score_cap) {
+    const int c = npairs + npairs / 4 + 1024;
+    if (L.seeds) (void)hipFree(L.seeds);
+    if (L.keys) (void)hipFree(L.keys);
+    if (L.idx) (void)hipFree(L.idx);
+    if (L.tkeys) (void)hipFree(L.tkeys);
+    if (L.tidx) (void)hipFree(L.tidx);
+    if (L.hist) (void)hipFree(L.hist);
+    if (L.part) (void)hipFree(L.part);
+    L.seeds = nullptr; L.keys = L.idx = L.tkeys = L.tidx = L.hist = L.part = nullptr;
+    L.score_cap = 0;
+    const size_t hw = radix_hist_words(c), pw = scan_partial_words((int)hw);
+    CS_HIP_RET(hipMalloc((void**)&L.seeds, sizeof(float) * (size_t)c * SUB_GROUP));
+    CS_HIP_RET(hipMalloc((void**)&L.keys, sizeof(uint32_t) * (size_t)c));
+    CS_HIP_RET(hipMalloc((void**)&L.idx, sizeof(uint32_t) * (size_t)c));
+    CS_HIP_RET(hipMalloc((void**)&L.tkeys, sizeof(uint32_t) * (size_t)c));
+    CS_HIP_RET(hipMalloc((void**)&L.tidx, sizeof(uint32_t) * (size_t)c));
+    CS_HIP_RET(hipMalloc((void**)&L.hist, sizeof(uint32_t) * hw));
+    CS_HIP_RET(hipMalloc((void**)&L.part, sizeof(uint32_t) * pw));
+    L.score_cap = c;
+  }
+  const int ntile = (npairs + 3) / 4;
+  hipLaunchKernelGGL(k_learned_score, dim3((unsigned)std::min(ntile, 4096)), dim3(MLP_THREADS), 0,
+                     st, pairs, npairs, npairs_dev, cap, S.ev, S.x_old, S.av, dt, h, L.wb, L.w,
+                     L.seeds, L.keys, L.idx);
+  CS_HIP_RET(hipGetLastError());
+  bool in_tmp = false;
+  CS_HIP_RET(radix_sort_u32(st, L.keys, L.idx, L.tkeys, L.tidx, L.hist, L.part, npairs, 32, &in_tmp));
+  *perm = in_tmp ? L.tidx : L.idx;
+  return hipSuccess;
 }
 
 // impact-zone merges of hit pairs carried from the previous pass: the points

@@ -1124,7 +1124,8 @@ __global__ void __launch_bounds__(SP_THREADS)
   }
 }
 
-hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps) {
+hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps,
+                        std::vector<hipEvent_t>* marks) {
   const int n = S.n_sched;
   if (S.ne == 0 || n == 0 || max_sweeps <= 0) return hipSuccess;
   // sweeps in flight: marks pending at one time cover K * delta - 1 levels
@@ -1136,6 +1137,9 @@ hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps) 
     CS_HIP_RET(hipMemsetAsync(S.lvl_active, 0, (size_t)S.n_levels, st));
     hipLaunchKernelGGL(k_strain_check, dim3((n + 255) / 256), dim3(256), 0, st, S.sslot, dt, n,
                        S.spre, S.lvl_active, S.counters);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (marks && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+      (void)hipEventRecord(e0, st);
     if (S.n_levels <= SP_LP_MAX)
       hipLaunchKernelGGL(k_strain_pipe<true>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
                          S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
@@ -1144,6 +1148,11 @@ hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps) 
       hipLaunchKernelGGL(k_strain_pipe<false>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
                          S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
                          S.counters);
+    if (e1) {
+      (void)hipEventRecord(e1, st);
+      marks->push_back(e0);
+      marks->push_back(e1);
+    }
     hipLaunchKernelGGL(k_strain_writeback, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.sslot,
                        S.slast, S.av, S.nv);
     k0 += K;

@@ -226,6 +226,64 @@ class CollisionSolver3d:
         self._check(self.L.cs_get_bodies(self.h, _p(com, C.c_double), _p(vel, C.c_double)))
         return com[:nb], vel[:nb]
 
+    # ---- learned time-of-impact proposer (no counterpart in the reference) ----
+    def setLearnedWeights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        if w.size != _lib.MLP_PARAMS:
+            raise ValueError(f"expected {_lib.MLP_PARAMS} parameters, got {w.size}")
+        self._check(self.L.cs_learned_set_weights(self.h, _p(w, C.c_float)))
+
+    def learnedWeights(self) -> np.ndarray:
+        w = np.empty(_lib.MLP_PARAMS, np.float32)
+        self._check(self.L.cs_learned_get_weights(self.h, _p(w, C.c_float)))
+        return w
+
+    def learnedInit(self, seed: int = 0):
+        self._check(self.L.cs_learned_init(self.h, seed))
+
+    def learnedInfer(self, x) -> np.ndarray:
+        """(logit, raw seed) of each row of x [n][24] on the device"""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 24)
+        out = np.empty((x.shape[0], 2), np.float32)
+        self._check(self.L.cs_learned_infer(self.h, _p(x, C.c_float), _p(out, C.c_float),
+                                            x.shape[0]))
+        return out
+
+    def learnedGrad(self, x, exists, toi):
+        """(mean loss, gradient) of one batch on the device"""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 24)
+        e = np.ascontiguousarray(exists, np.float32)
+        t = np.ascontiguousarray(toi, np.float32)
+        g = np.empty(_lib.MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(self.L.cs_learned_grad(self.h, _p(x, C.c_float), _p(e, C.c_float),
+                                           _p(t, C.c_float), x.shape[0], _p(g, C.c_float),
+                                           C.byref(loss)))
+        return float(loss.value), g
+
+    def learnedCollect(self, capacity: int, stride: int = 1):
+        self._check(self.L.cs_learned_collect(self.h, capacity, stride))
+
+    def learnedDataset(self):
+        n = self.L.cs_learned_dataset_size(self.h)
+        if n < 0:
+            raise CollisionError(_lib.CS_ERR_HIP, "dataset size")
+        x = np.empty((n, 24), np.float32)
+        e, t = np.empty(n, np.float32), np.empty(n, np.float32)
+        self._check(self.L.cs_learned_dataset(self.h, _p(x, C.c_float), _p(e, C.c_float),
+                                              _p(t, C.c_float), n))
+        return x, e, t
+
+    def learnedTrain(self, epochs: int, batch: int = 8192, lr: float = 1e-3, seed: int = 0):
+        loss = np.empty(max(epochs, 1), np.float32)
+        self._check(self.L.cs_learned_train(self.h, epochs, batch, lr, seed, _p(loss, C.c_float)))
+        return loss[:epochs]
+
+    def learnedStats(self) -> dict:
+        st = _lib.LearnedStats()
+        self._check(self.L.cs_learned_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
 
 def math_eval(fn: str, x, device: int = 0) -> np.ndarray:
     """The step's own acos / cos / sin / pow(x, 1/3), evaluated on the device."""

@@ -69,6 +69,18 @@ typedef struct cs_params {
                                    collsn_num of the last pass with a hit (cs_last_impulses,
                                    cs_write_vtk); the reference's CollisionImpulse
                                    debugging dump (dcollid.cpp:684-694) */
+    int32_t precision;          /* 0: FP64 throughout (the plain form); 1: mixed -- the
+                                   proximity pass screens every sub-test in FP32 from a
+                                   local origin with certified forward-error bounds and
+                                   runs the FP64 test only where the screen cannot
+                                   prove a reject (results identical to 0) */
+    int32_t learned;            /* 1: the learned CCD form (cs_learned_*): an MLP scores
+                                   every CCD sub-test and seeds a bracketed Newton
+                                   solve; the exact cubic still decides */
+    int32_t precision_stats;    /* 1: with precision = 1, also evaluate the FP64
+                                   quantities of every screened sub-test and report the
+                                   largest FP32 error (cs_step_report.mx_*) */
+    int32_t pad1;
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
@@ -133,6 +145,17 @@ typedef struct cs_step_report {
     int64_t hit_pairs, hit_subtests;
     double ms_total, ms_broad, ms_narrow, ms_response, ms_zone, ms_strain;
     int64_t strain_slots;    /* element visits the strain sweeps evaluated (sum) */
+    double ms_strain_kernel; /* strain-sweep kernel time (pass_timing >= 1) */
+    int64_t host_syncs;      /* host synchronisations of the resolve */
+    /* mixed precision (precision = 1): proximity sub-tests the FP32 screen
+     * rejected, and those it passed on to FP64; with precision_stats the
+     * largest FP32 error over the screened sub-tests of the distance (relative
+     * to the thickness) and of the barycentric / segment parameters, and the
+     * largest ratio of a screened quantity's FP32 error to the forward-error
+     * bound the screen certified it with (at most 1 when the bounds hold) */
+    int64_t mx_screened, mx_rechecked;
+    double mx_err_dist, mx_err_param;
+    double mx_bound_ratio;
 } cs_step_report;
 
 typedef struct cs_solver cs_solver;
@@ -198,6 +221,58 @@ int cs_zone_labels_noalloc(cs_solver *h, int32_t *out);
  * the ones isCoplanar and the impact-zone rotation use (dcollid3d.cpp:163-169,
  * 436-444); host arrays.  A parity tap with no counterpart in the reference. */
 int cs_math_eval(int32_t device, int32_t fn, const double *x, double *y, int64_t n);
+
+/* ---- learned time-of-impact proposer (no counterpart in the reference) ----
+ * An MLP 24 -> 128 -> 128 -> 2 (ReLU; BF16 weights and activations, FP32
+ * accumulation on the matrix cores) maps a CCD sub-test to a root-exists
+ * logit and a time-of-impact seed in [0, 1] (fraction of dt).  The 24
+ * features are the four points' x_old relative to their centroid and their
+ * avgVel * dt, both divided by the thickness.  Labels come from the plain
+ * CCD itself: root-exists = isCoplanar true (dcollid3d.cpp:371-482), seed =
+ * first valid root / dt.  With cs_params.learned = 1 every CCD pass scores
+ * its sub-tests, tests its pairs in order of decreasing score and runs a
+ * safeguarded bracketed Newton solve of the coplanarity cubic from the seed
+ * next to the exact closed form (which still decides; results are those of
+ * the plain form bit for bit); cs_learned_stats reports the outcome. */
+#define CS_MLP_PARAMS 19970   /* 24*128 + 128 + 128*128 + 128 + 128*2 + 2 */
+typedef struct cs_learned_stats {
+    int64_t scored;            /* sub-tests scored by the network */
+    int64_t exists;            /* of those, with a valid root (exact cubic) */
+    int64_t predicted_exists;  /* score > 0 */
+    int64_t agree;             /* prediction == exact outcome */
+    int64_t newton_roots;      /* roots the seeded Newton solve located */
+    int64_t newton_iters;      /* its iterations (seeded at the network's TOI) */
+    int64_t newton_iters_mid;  /* iterations of the same solve seeded at dt / 2 */
+    double max_root_err;       /* largest |Newton root - closed-form root| / dt */
+} cs_learned_stats;
+/* parameters, FP32 master copy, in the order W1[128][24], b1[128],
+ * W2[128][128], b2[128], W3[2][128], b3[2] (row = output unit) */
+int cs_learned_set_weights(cs_solver *h, const float *w);
+int cs_learned_get_weights(cs_solver *h, float *w);
+/* random initialisation (He uniform, seeded) */
+int cs_learned_init(cs_solver *h, uint64_t seed);
+/* forward pass on host features x[n][24]: out[n][2] = (logit, seed) */
+int cs_learned_infer(cs_solver *h, const float *x, float *out, int64_t n);
+/* loss and gradient of the training objective on one batch (mean over the
+ * batch of BCE(logit, exists) + exists * (sigmoid(s) - toi)^2), for checks */
+int cs_learned_grad(cs_solver *h, const float *x, const float *exists, const float *toi,
+                    int64_t n, float *grad, float *loss);
+/* from now on, every CCD pass of cs_resolve appends (up to `capacity` in
+ * total) sub-tests with their labels to the handle's dataset, keeping those
+ * whose hash of (pair, sub-test, pass) is divisible by `stride`; a call
+ * empties the dataset, capacity 0 stops collecting and frees it.  The
+ * samples' order in the dataset is that of the device's atomics; training
+ * puts them in a canonical order first. */
+int cs_learned_collect(cs_solver *h, int64_t capacity, int32_t stride);
+int64_t cs_learned_dataset_size(cs_solver *h);
+/* copy the dataset out: x[n][24] (float), exists[n], toi[n] */
+int cs_learned_dataset(cs_solver *h, float *x, float *exists, float *toi, int64_t cap);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over shuffled batches of the
+ * collected dataset, all on the device; loss[i] = mean loss of epoch i */
+int cs_learned_train(cs_solver *h, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                     float *loss);
+/* statistics of the learned form since the previous call (which resets them) */
+int cs_learned_get_stats(cs_solver *h, cs_learned_stats *out);
 
 /* ---- diagnostics and restart ---- */
 /* the summed collsnImpulse [nv][3] and collsn_num [nv] every vertex received

@@ -26,12 +26,15 @@ def sorted_log(log):
 
 
 def run_parity(scene, steps, check_every=1, compare_zones=True, verbose=False, gpu_kw=None,
-               gpu_log_pairs=1, **kw):
+               gpu_log_pairs=1, gpu_setup=None, **kw):
     """gpu_log_pairs=0 runs the library as shipped (no parity taps: the pair
     stream is not carried between incremental passes and their candidate and
     pair counts are -1); the hit counts, zones and the full state are still
-    compared every step (check_every only applies with the taps on)."""
+    compared every step (check_every only applies with the taps on).
+    gpu_setup(solver) runs once after the library's solver is made."""
     gpu = solver_for_scene(scene, log_pairs=gpu_log_pairs, **(gpu_kw or {}), **kw)
+    if gpu_setup is not None:
+        gpu_setup(gpu)
     orc = O.OracleSolver(scene, log_pairs=1, order_mode=O.ORDER_CANONICAL,
                          math_mode=O.MATH_DET, **kw)
     stats = dict(steps=0, hit_pairs=0, zone_steps=0, strain_sweeps=0)

@@ -326,3 +326,65 @@ def test_vtk_dump_of_the_last_pass(tmp_path):
     assert np.array_equal(col[:, 0] == 255, red) and np.array_equal(col[:, 1] == 255, ~red)
     gpu.close()
     orc.close()
+
+
+# ---------------------------------------------------------------------------
+# mixed precision: FP32 proximity screen + certified fast coplanarity screen,
+# both exact by construction (every uncertain sub-test runs the FP64 path)
+@pytest.mark.parametrize("maker,steps", [
+    (lambda: S.cloth_over_sphere(), 64),
+    (lambda: S.self_collision_cloth(n=64, steps=12), 12),
+    (lambda: S.combined_scene(n=96, nstrings=60, steps=60), 60),
+    (S.fan_disc_on_box, 8), (S.deck_string_string, 120), (S.deck_box_boundary, 150),
+], ids=["config0_contact", "config1_64", "config4_96", "fan_disc", "string_string", "box_boundary"])
+def test_mixed_precision_form_is_bitwise_plain(maker, steps):
+    sc = maker()
+    if sc.name == "config0_cloth_sphere":
+        sc.x = sc.x + sc.vel * (sc.dt * 236)
+    st = run_parity(sc, steps=steps, verbose=True, gpu_kw=dict(precision=1, precision_stats=1))
+    assert st["hit_pairs"] > 0
+
+
+def test_mixed_precision_screens_and_reports_errors():
+    """The FP32 screen rejects most proximity sub-tests of config 4 (reduced);
+    every screened quantity's FP32 error stays within the forward-error bound
+    the screen certified it with, and the largest raw errors are reported
+    (the line distance of nearly parallel edges can be far off in FP32 --
+    with a bound to match, so such a pair is passed on to FP64)."""
+    from collision_amd.solver import solver_for_scene
+    sc = S.combined_scene(n=96, nstrings=60, steps=20)
+    g = solver_for_scene(sc, precision=1, precision_stats=1)
+    scr = chk = 0
+    ed = ep = er = 0.0
+    for _ in range(20):
+        r = g.step()
+        scr += r.mx_screened
+        chk += r.mx_rechecked
+        ed, ep = max(ed, r.mx_err_dist), max(ep, r.mx_err_param)
+        er = max(er, r.mx_bound_ratio)
+    g.close()
+    print(f"[mixed] screened {scr} rechecked {chk} err_dist/h {ed:.3e} err_param {ep:.3e} "
+          f"error/bound {er:.3e}")
+    assert scr > 4 * chk
+    assert ed > 0 and ep > 0
+    assert 0 < er <= 1.0
+
+
+def test_config1_full_size_mixed_form_against_golden():
+    import json
+    import os
+
+    from collision_amd.solver import solver_for_scene
+    from tests.golden.make_step_golden import REPORT_KEYS, WORKLOADS, state_hash
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "steps_config1_256.json")
+    gold = json.load(open(path))
+    sc = WORKLOADS["config1_256"][0]()
+    g = solver_for_scene(sc, precision=1)
+    try:
+        for k, rec in enumerate(gold["records"]):
+            r = g.step().as_dict()
+            keys = [key for key in REPORT_KEYS if key not in ("candidates", "narrow_pairs")]
+            assert {key: int(r[key]) for key in keys} == {key: rec["report"][key] for key in keys}, k
+            assert state_hash(g.getState()) == rec["state"], k
+    finally:
+        g.close()
