@@ -66,8 +66,8 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=10,
                     help="config-1 (256x256) plain steps whose CCD sub-tests train the network")
     ap.add_argument("--train-stride", type=int, default=16)
-    ap.add_argument("--train-samples", type=int, default=8_000_000)
-    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--train-samples", type=int, default=4_000_000)
+    ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device copies (scale for HBM byte counters)")
     return ap.parse_args()
@@ -139,7 +139,12 @@ def train_network(args):
     g.learnedCollect(capacity=args.train_samples, stride=args.train_stride)
     for _ in range(args.train_steps):
         g.step()
-    n = int(g.L.cs_learned_dataset_size(g.h))
+    _, ex, _ = g.learnedDataset()
+    n = len(ex)
+    npos = int((ex > 0.5).sum())
+    # each epoch repeats the samples with a root round(#without / #with) times
+    rep = min(64, max(1, (n - npos + npos // 2) // npos)) if npos else 1
+    m = n - npos + rep * npos
     g.synchronize()
     t0 = time.perf_counter()
     losses = g.learnedTrain(epochs=args.epochs, batch=8192, lr=1e-3, seed=1)
@@ -147,12 +152,13 @@ def train_network(args):
     tt = time.perf_counter() - t0
     w = g.learnedWeights()
     g.close()
-    flops = MLP_TRAIN_FLOP * n * args.epochs
-    info = {"samples": n, "epochs": args.epochs, "batch": 8192, "lr": 1e-3,
+    flops = MLP_TRAIN_FLOP * m * args.epochs
+    info = {"samples": n, "samples_with_root": npos, "samples_per_epoch": m,
+            "epochs": args.epochs, "batch": 8192, "lr": 1e-3,
             "source": f"config 1 (256x256) plain CCD sub-tests, steps 1..{args.train_steps}, "
                       f"1 in {args.train_stride} kept",
             "epoch_losses": [float(v) for v in losses], "seconds": tt,
-            "samples_per_s": n * args.epochs / tt if tt > 0 else None,
+            "samples_per_s": m * args.epochs / tt if tt > 0 else None,
             "tflops": flops / tt / 1e12 if tt > 0 else None,
             "mfma_frac": flops / tt / 1e12 / MFMA_BF16_PEAK_TFLOPS if tt > 0 else None,
             "flop_note": f"{MLP_TRAIN_FLOP} algorithmic FLOP per sample (forward + backward), "
@@ -189,7 +195,10 @@ def run_forms(sc, weights):
         if kw.get("learned"):
             ls = s.learnedStats()
             roots = max(1, ls["newton_roots"])
+            tp = (ls["predicted_exists"] + ls["exists"] - ls["scored"] + ls["agree"]) // 2
             r["learned"] = dict(ls, accuracy=ls["agree"] / max(1, ls["scored"]),
+                                recall=tp / max(1, ls["exists"]),
+                                precision=tp / max(1, ls["predicted_exists"]),
                                 newton_iters_per_root=ls["newton_iters"] / roots,
                                 midpoint_iters_per_root=ls["newton_iters_mid"] / roots)
         s.close()

@@ -149,7 +149,7 @@ void free_dev(DevState& S) {
   dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
   dfree(S.zkey); dfree(S.zval); dfree(S.zval_tmp);
   dfree(S.zhead); dfree(S.zend);
-  dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters);
+  dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters); dfree(S.spread);
   dfree(S.sslot); dfree(S.spre); dfree(S.slast);
   dfree(S.lvl_active);
   S = DevState();
@@ -304,6 +304,14 @@ hipError_t narrow_call(cs_solver* h, NarrowParams NP, const int2* pairs, int npa
                                     NP.thickness > 0 ? NP.thickness : 1.0, &perm));
     NP.lseed = h->L.seeds;
     NP.perm = perm;
+    // A pair whose sub-tests the certified screen all rejected has no hit
+    // and merges no zone (merges start at a pair's first hit), so the narrow
+    // phase only runs the pairs with a scored slot, which the sort put first.
+    // (Dataset collection needs every sub-test's label: then all pairs run.)
+    if (NP.ds_cap == 0) {
+      launch_narrow(h->st, h->S, NP, pairs, npairs, h->L.count + 1, npairs);
+      return hipGetLastError();
+    }
   }
   launch_narrow(h->st, h->S, NP, pairs, npairs, npairs_dev, cap);
   return hipGetLastError();
@@ -364,6 +372,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
   NP.zone_mode = (kind == CS_PASS_ZONE);
   NP.log_pairs = P.log_pairs;
   NP.pass_id = pass_id;
+  NP.spread = S.spread;
   if (ccd && h->L.ds_cap > 0) {  // dataset collection (cs_learned_collect)
     NP.dsx = h->L.dsx;
     NP.dse = h->L.dse;
@@ -978,6 +987,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   S.plog_cap = 1 << 20;
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
   HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
+  HIPCHK(h, dalloc(&S.spread, SPREAD_SLOTS * SPREAD_STRIDE));
   HIPCHK(h, alloc_sort_hist(S));
   HIPCHK(h, hipMemcpy(S.ev, ev.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.eflags, ef.data(), ne, hipMemcpyHostToDevice));
@@ -1138,6 +1148,9 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   // still in the counter block
   if (S.ne >= 2) launch_clear_ranges(h->st, S);
   HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
+  const bool spread_stats = P.precision == 1 || P.learned;
+  if (spread_stats)
+    HIPCHK(h, hipMemsetAsync(S.spread, 0, sizeof(int) * SPREAD_SLOTS * SPREAD_STRIDE, h->st));
   launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
   if (P.record_impulses) {  // the impulses of this resolve's last pass with a hit
     if (!S.pimp) {
@@ -1236,33 +1249,40 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   for (size_t i = 0; i + 1 < strain_marks.size(); i += 2)
     R.ms_strain_kernel += elapsed(strain_marks[i], strain_marks[i + 1]);
   for (hipEvent_t e : strain_marks) (void)hipEventDestroy(e);
+  if (spread_stats) {  // the statistics' per-work-group copies: sums and maxima
+    std::vector<int> sp((size_t)SPREAD_SLOTS * SPREAD_STRIDE);
+    HIPCHK(h, hipMemcpyAsync(sp.data(), S.spread, sizeof(int) * sp.size(), hipMemcpyDeviceToHost,
+                             h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->syncs++;
+    int64_t sum[SP_COUNT] = {};
+    float mx[SP_COUNT] = {};
+    for (int k = 0; k < SPREAD_SLOTS; ++k)
+      for (int c = 0; c < SP_COUNT; ++c) {
+        const int v = sp[(size_t)k * SPREAD_STRIDE + c];
+        sum[c] += v;
+        float f;
+        std::memcpy(&f, &v, sizeof(float));
+        mx[c] = std::max(mx[c], f);
+      }
+    R.mx_screened = sum[SP_MX_SCREENED];
+    R.mx_rechecked = sum[SP_MX_RECHECK];
+    R.mx_err_dist = mx[SP_MX_EDIST];
+    R.mx_err_param = mx[SP_MX_EPARAM];
+    R.mx_bound_ratio = mx[SP_MX_RATIO];
+    if (P.learned) {
+      cs_learned_stats& ls = h->lstats;
+      ls.scored += sum[SP_LN_SCORED];
+      ls.exists += sum[SP_LN_EXISTS];
+      ls.predicted_exists += sum[SP_LN_PRED];
+      ls.agree += sum[SP_LN_AGREE];
+      ls.newton_roots += sum[SP_LN_ROOTS];
+      ls.newton_iters += sum[SP_LN_ITERS];
+      ls.newton_iters_mid += sum[SP_LN_ITERS_MID];
+      ls.max_root_err = std::max(ls.max_root_err, (double)mx[SP_LN_EMAX]);
+    }
+  }
   R.host_syncs = h->syncs;
-  R.mx_screened = h->h_counters[CNT_MX_SCREENED];
-  R.mx_rechecked = h->h_counters[CNT_MX_RECHECK];
-  {
-    float ed, ep;
-    std::memcpy(&ed, &h->h_counters[CNT_MX_EDIST], sizeof(float));
-    std::memcpy(&ep, &h->h_counters[CNT_MX_EPARAM], sizeof(float));
-    float er;
-    std::memcpy(&er, &h->h_counters[CNT_MX_RATIO], sizeof(float));
-    R.mx_err_dist = ed;
-    R.mx_err_param = ep;
-    R.mx_bound_ratio = er;
-  }
-  if (P.learned) {
-    const int* c = h->h_counters;
-    cs_learned_stats& ls = h->lstats;
-    ls.scored += c[CNT_LN_SCORED];
-    ls.exists += c[CNT_LN_EXISTS];
-    ls.predicted_exists += c[CNT_LN_PRED];
-    ls.agree += c[CNT_LN_AGREE];
-    ls.newton_roots += c[CNT_LN_ROOTS];
-    ls.newton_iters += c[CNT_LN_ITERS];
-    ls.newton_iters_mid += c[CNT_LN_ITERS_MID];
-    float em;
-    std::memcpy(&em, &c[CNT_LN_EMAX], sizeof(float));
-    ls.max_root_err = std::max(ls.max_root_err, (double)em);
-  }
   if (report) *report = R;
   return status;
 }

@@ -52,20 +52,7 @@ enum {
   CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
   CNT_PCAND64 = 32,      // 32, 33: static-box overlaps found by the swept-box traversal
   CNT_PPAIRS = 34,       // pairs of that traversal whose static boxes overlap
-  CNT_MX_SCREENED = 35,  // mixed precision: sub-tests the FP32 screen rejected (whole step)
-  CNT_MX_RECHECK = 36,   // sub-tests it passed on to FP64
-  CNT_MX_EDIST = 37,     // largest FP32 distance error / thickness (float bits)
-  CNT_MX_EPARAM = 38,    // largest FP32 barycentric / segment parameter error (float bits)
-  CNT_MX_RATIO = 39,     // largest FP32 error / its certified bound (float bits)
-  CNT_LN_SCORED = 40,    // learned form (whole step): CCD sub-tests scored by the network
-  CNT_LN_EXISTS = 41,    // of those, isCoplanar true (exact cubic)
-  CNT_LN_PRED = 42,      // predicted to have a root (logit > 0)
-  CNT_LN_AGREE = 43,     // prediction == exact outcome
-  CNT_LN_ROOTS = 44,     // roots located by the seeded Newton solve
-  CNT_LN_ITERS = 45,     // its iterations, seeded at the network's time of impact
-  CNT_LN_ITERS_MID = 46, // iterations of the same solve seeded at dt / 2
-  CNT_LN_EMAX = 47,      // largest |Newton root - closed-form root| / dt (float bits)
-  CNT_INTS = 48
+  CNT_INTS = 40
 };
 
 struct NarrowParams {
@@ -78,6 +65,7 @@ struct NarrowParams {
   int zone_mode;  // merge impact zones (s_detImpZone)
   int log_pairs;
   int pass_id;
+  int* spread;  // [SPREAD_SLOTS][SPREAD_STRIDE] statistics (SP_*)
   // learned form (CCD passes): the scoring kernel's per-sub-test seeds
   // (16 per pair, by pair position before reordering; sign bit set = root
   // predicted) and the reordering of the pairs (nullptr: none)
@@ -92,6 +80,29 @@ struct NarrowParams {
   long long ds_cap;
   int ds_stride;
   uint32_t ds_salt;
+};
+
+// Statistics of the mixed-precision screens and the learned form, summed by
+// many wavefronts: each work-group adds into one of SPREAD_SLOTS copies (its
+// block index modulo the count, one 128-byte line each), so that no single
+// address takes an atomic from every wavefront; the host adds the copies
+// (sums) or takes their maximum (float bits of non-negative values).
+constexpr int SPREAD_SLOTS = 64, SPREAD_STRIDE = 32;
+enum {
+  SP_MX_SCREENED = 0,  // proximity sub-tests the FP32 screen rejected
+  SP_MX_RECHECK,       // sub-tests it passed on to FP64
+  SP_MX_EDIST,         // max FP32 distance error / thickness (float bits)
+  SP_MX_EPARAM,        // max FP32 barycentric / segment parameter error (float bits)
+  SP_MX_RATIO,         // max FP32 error / its certified bound (float bits)
+  SP_LN_SCORED,        // learned form: CCD sub-tests scored by the network
+  SP_LN_EXISTS,        // of those, isCoplanar true (exact cubic)
+  SP_LN_PRED,          // predicted to have a root (logit > 0)
+  SP_LN_AGREE,         // prediction == exact outcome
+  SP_LN_ROOTS,         // roots located by the seeded Newton solve
+  SP_LN_ITERS,         // its iterations, seeded at the network's time of impact
+  SP_LN_ITERS_MID,     // iterations of the same solve seeded at dt / 2
+  SP_LN_EMAX,          // max |Newton root - closed-form root| / dt (float bits)
+  SP_COUNT
 };
 
 // One slot of the strain-limit schedule (one non-rigid element), 256 B.
@@ -221,6 +232,7 @@ struct DevState {
   int plog_cap = 0;
   // counters / flags
   int* counters = nullptr;
+  int* spread = nullptr;  // [SPREAD_SLOTS][SPREAD_STRIDE] statistics (SP_*)
 };
 
 // The time-of-impact proposer of a handle: FP32 master weights, Adam state,
@@ -231,8 +243,11 @@ struct LearnedState {
   uint16_t* wb = nullptr;
   bool ready = false;  // weights set
   int64_t adam_t = 0;
-  // learned form: seeds [16 * cap], sort keys / positions and their partners
+  // learned form: per-slot seed codes [16 * cap], the list of slots to
+  // score and its length, pair sort keys / positions and their partners
   float* seeds = nullptr;
+  uint32_t* list = nullptr;
+  int* count = nullptr;
   uint32_t *keys = nullptr, *idx = nullptr, *tkeys = nullptr, *tidx = nullptr;
   uint32_t *hist = nullptr, *part = nullptr;
   int score_cap = 0;

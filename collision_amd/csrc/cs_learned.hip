@@ -312,7 +312,8 @@ hipError_t learned_alloc(LearnedState& L) {
 void learned_free(LearnedState& L) {
   lfree(L.w); lfree(L.m); lfree(L.v); lfree(L.grad); lfree(L.wb);
   lfree(L.ds_count); lfree(L.dsx); lfree(L.dse); lfree(L.dst);
-  lfree(L.seeds); lfree(L.keys); lfree(L.idx); lfree(L.tkeys); lfree(L.tidx);
+  lfree(L.seeds); lfree(L.list); lfree(L.count);
+  lfree(L.keys); lfree(L.idx); lfree(L.tkeys); lfree(L.tidx);
   lfree(L.hist); lfree(L.part); lfree(L.gpart); lfree(L.lpart);
   L = LearnedState();
 }
@@ -405,38 +406,65 @@ int64_t learned_dataset_size(LearnedState& L, hipStream_t st) {
 }
 
 // Adam over seeded shuffles of the collected dataset; loss[e] = mean of the
-// batch losses of epoch e
+// batch losses of epoch e.  The classes are balanced by repetition: every
+// sample with a root appears r = round(#without / #with) times per epoch
+// (1 <= r <= 64), so both labels weigh about the same in the loss.
 hipError_t learned_train(LearnedState& L, hipStream_t st, int epochs, int batch, float lr,
                          uint64_t seed, float* host_loss) {
   const int64_t n64 = learned_dataset_size(L, st);
-  if (n64 <= 0 || n64 > INT32_MAX / 2 || batch <= 0 || epochs < 0) return hipErrorInvalidValue;
+  if (n64 <= 0 || n64 > INT32_MAX / 4 || batch <= 0 || epochs < 0) return hipErrorInvalidValue;
   const int n = (int)n64;
-  const int nbatch = (n + batch - 1) / batch;
-  const size_t hw = radix_hist_words(n), pw = scan_partial_words((int)hw);
   uint64_t *k64 = nullptr, *k64t = nullptr;
   uint32_t *order = nullptr, *ordt = nullptr, *k32 = nullptr, *k32t = nullptr, *bidx = nullptr,
-           *bidxt = nullptr, *hist = nullptr, *part = nullptr;
+           *bidxt = nullptr, *hist = nullptr, *part = nullptr, *ext = nullptr;
   float* bloss = nullptr;
-  std::vector<float> bl((size_t)nbatch);
   bool in_tmp = false;
   hipError_t e = hipSuccess;
   do {
+    // canonical order of the samples (a hash of their contents)
+    const size_t hw0 = radix_hist_words(n), pw0 = scan_partial_words((int)hw0);
     if ((e = lalloc(&k64, n)) || (e = lalloc(&k64t, n)) || (e = lalloc(&order, n)) ||
-        (e = lalloc(&ordt, n)) || (e = lalloc(&k32, n)) || (e = lalloc(&k32t, n)) ||
-        (e = lalloc(&bidx, n)) || (e = lalloc(&bidxt, n)) || (e = lalloc(&hist, hw)) ||
-        (e = lalloc(&part, pw)) || (e = lalloc(&bloss, nbatch)))
+        (e = lalloc(&ordt, n)) || (e = lalloc(&hist, hw0)) || (e = lalloc(&part, pw0)))
       break;
     hipLaunchKernelGGL(k_ds_keys, dim3((n + 255) / 256), dim3(256), 0, st, L.dsx, L.dse, L.dst,
                        n, k64, order);
     if ((e = radix_sort_u64(st, k64, order, k64t, ordt, hist, part, n, 64, &in_tmp))) break;
-    const uint32_t* ord = in_tmp ? ordt : order;
+    std::vector<uint32_t> ord((size_t)n);
+    std::vector<float> lab((size_t)n);
+    if ((e = hipMemcpyAsync(ord.data(), in_tmp ? ordt : order, sizeof(uint32_t) * n,
+                            hipMemcpyDeviceToHost, st)))
+      break;
+    if ((e = hipMemcpyAsync(lab.data(), L.dse, sizeof(float) * n, hipMemcpyDeviceToHost, st))) break;
+    if ((e = hipStreamSynchronize(st))) break;
+    int64_t npos = 0;
+    for (float v : lab) npos += v > 0.5f;
+    const int64_t nneg = n - npos;
+    const int r = npos > 0 ? (int)std::min<int64_t>(64, std::max<int64_t>(1, (nneg + npos / 2) / npos)) : 1;
+    std::vector<uint32_t> ext_h;
+    ext_h.reserve((size_t)(nneg + r * npos));
+    for (uint32_t i : ord) {
+      const int rep = lab[i] > 0.5f ? r : 1;
+      for (int k = 0; k < rep; ++k) ext_h.push_back(i);
+    }
+    const int m = (int)ext_h.size();
+    const int nbatch = (m + batch - 1) / batch;
+    lfree(hist);
+    lfree(part);
+    const size_t hw = radix_hist_words(m), pw = scan_partial_words((int)hw);
+    if ((e = lalloc(&ext, m)) || (e = lalloc(&k32, m)) || (e = lalloc(&k32t, m)) ||
+        (e = lalloc(&bidx, m)) || (e = lalloc(&bidxt, m)) || (e = lalloc(&hist, hw)) ||
+        (e = lalloc(&part, pw)) || (e = lalloc(&bloss, nbatch)))
+      break;
+    if ((e = hipMemcpyAsync(ext, ext_h.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice, st)))
+      break;
+    std::vector<float> bl((size_t)nbatch);
     for (int ep = 0; ep < epochs && e == hipSuccess; ++ep) {
-      hipLaunchKernelGGL(k_shuffle_keys, dim3((n + 255) / 256), dim3(256), 0, st, ord, n,
+      hipLaunchKernelGGL(k_shuffle_keys, dim3((m + 255) / 256), dim3(256), 0, st, ext, m,
                          mix64(seed + 0x1000003ull * (uint64_t)(ep + 1)), k32, bidx);
-      if ((e = radix_sort_u32(st, k32, bidx, k32t, bidxt, hist, part, n, 32, &in_tmp))) break;
+      if ((e = radix_sort_u32(st, k32, bidx, k32t, bidxt, hist, part, m, 32, &in_tmp))) break;
       const uint32_t* bi = in_tmp ? bidxt : bidx;
       for (int b = 0; b < nbatch && e == hipSuccess; ++b) {
-        const int bs = std::min(batch, n - b * batch);
+        const int bs = std::min(batch, m - b * batch);
         e = learned_grad(L, st, L.dsx, L.dse, L.dst, bi + (int64_t)b * batch, bs, bloss + b);
         if (e == hipSuccess) e = learned_adam_step(L, st, lr);
       }
@@ -444,14 +472,14 @@ hipError_t learned_train(LearnedState& L, hipStream_t st, int epochs, int batch,
       if ((e = hipMemcpyAsync(bl.data(), bloss, sizeof(float) * nbatch, hipMemcpyDeviceToHost, st)))
         break;
       if ((e = hipStreamSynchronize(st))) break;
-      double s = 0;
-      for (float v : bl) s += v;
-      if (host_loss) host_loss[ep] = (float)(s / nbatch);
+      double sum = 0;
+      for (float v : bl) sum += v;
+      if (host_loss) host_loss[ep] = (float)(sum / nbatch);
     }
   } while (false);
   (void)hipStreamSynchronize(st);
   lfree(k64); lfree(k64t); lfree(order); lfree(ordt); lfree(k32); lfree(k32t);
-  lfree(bidx); lfree(bidxt); lfree(hist); lfree(part); lfree(bloss);
+  lfree(bidx); lfree(bidxt); lfree(hist); lfree(part); lfree(bloss); lfree(ext);
   return e;
 }
 

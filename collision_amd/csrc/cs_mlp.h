@@ -169,33 +169,34 @@ __device__ inline void cubic_coeffs(const double xo[4][3], const double vs[4][3]
   *d = x[3][0] * xx[0] - x[3][1] * xx[1] + x[3][2] * xx[2];
 }
 
-// First root of a t^3 + b t^2 + c t + d in [0, dt].  The interval is cut at
+// First root of a t^3 + b t^2 + c t + d in [t0, t1].  The interval is cut at
 // the critical points into monotone pieces; the first piece whose ends differ
 // in sign (or hold a zero) is the bracket.  Newton steps start from `seed`
 // (the bracket's midpoint when the seed lies outside it) and fall back to
 // bisection whenever they leave the bracket, which shrinks with every step;
-// the solve stops when a step moves less than 1e-15 dt or the value is zero.
-// Returns the iterations, or -1 when no piece changes sign.
-__device__ inline int newton_first_root(double a, double b, double c, double d, double dt,
-                                        double seed, double* root) {
+// the solve stops when a step moves less than 1e-15 (t1 - t0) or the value is
+// zero.  Returns the iterations, or -1 when no piece changes sign.
+__device__ inline int newton_first_root(double a, double b, double c, double d, double t0,
+                                        double t1, double seed, double* root) {
+  const double dt = t1 - t0;
   double cut[4];
   int nc = 0;
-  cut[nc++] = 0.0;
+  cut[nc++] = t0;
   const double qa = 3.0 * a, qb = 2.0 * b, qc = c;
   if (qa != 0.0) {
     const double disc = qb * qb - 4.0 * qa * qc;
     if (disc > 0.0) {
       const double s = sqrt(disc);
-      double t1 = (-qb - s) / (2.0 * qa), t2 = (-qb + s) / (2.0 * qa);
-      if (t1 > t2) { const double t = t1; t1 = t2; t2 = t; }
-      if (t1 > 0.0 && t1 < dt) cut[nc++] = t1;
-      if (t2 > 0.0 && t2 < dt) cut[nc++] = t2;
+      double r1 = (-qb - s) / (2.0 * qa), r2 = (-qb + s) / (2.0 * qa);
+      if (r1 > r2) { const double t = r1; r1 = r2; r2 = t; }
+      if (r1 > t0 && r1 < t1) cut[nc++] = r1;
+      if (r2 > t0 && r2 < t1) cut[nc++] = r2;
     }
   } else if (qb != 0.0) {
-    const double t1 = -qc / qb;
-    if (t1 > 0.0 && t1 < dt) cut[nc++] = t1;
+    const double r1 = -qc / qb;
+    if (r1 > t0 && r1 < t1) cut[nc++] = r1;
   }
-  cut[nc++] = dt;
+  cut[nc++] = t1;
   for (int p = 0; p + 1 < nc; ++p) {
     double lo = cut[p], hi = cut[p + 1];
     const double flo = ((a * lo + b) * lo + c) * lo + d;

@@ -27,6 +27,10 @@ namespace cs {
 
 constexpr int NARROW_THREADS = 128;
 constexpr int SUB_GROUP = 16;  // lanes per pair (sub-tests 0..14)
+// learned form: codes of the per-slot seeds (a scored slot holds its seed in
+// [0, 1], sign bit set when a root is predicted) and the 8-bit pair keys
+constexpr float SEED_CERTAIN = 4.0f, SEED_INACTIVE = 8.0f;
+constexpr uint32_t KEY_NONE = 255u;  // 0 = best score .. 254; 255 = no scored slot
 
 struct Sub {
   double xo[4][3];
@@ -346,7 +350,8 @@ __device__ bool d_edge_to_edge(const Sub& S, const double c[4][3], double h, con
 // beyond the validity window (2 MACH_EPS, dt + MACH_EPS] by more than the
 // margin, and the Cardano branch's two thresholds on |A| and |A - B| must be
 // cleared by the margin as well.  Anything else -- NaN included -- is not
-// certain and runs the exact path.
+// certain and runs the exact path.  When |a| <= MACH_EPS (the quadratic and
+// linear branch, no transcendental) the screen evaluates isCoplanar exactly.
 __device__ bool d_coplanar_certainly_false(const Sub& S, double dt) {
   double v[4][3], x[4][3];
 #pragma unroll
@@ -372,7 +377,27 @@ __device__ bool d_coplanar_certainly_false(const Sub& S, double dt) {
   c = x[3][0] * vx[0] - x[3][1] * vx[1] + x[3][2] * vx[2] + v[3][0] * xx[0] - v[3][1] * xx[1] +
       v[3][2] * xx[2];
   d = x[3][0] * xx[0] - x[3][1] * xx[1] + x[3][2] * xx[2];
-  if (!(fabs(a) > CS_MACH_EPS)) return false;  // the quadratic / linear branch is exact and cheap
+  if (!(fabs(a) > CS_MACH_EPS)) {
+    // the quadratic / linear branch has no transcendental: run it exactly as
+    // isCoplanar does (same operations, so the verdict is exact)
+    double r[3] = {-1.0, -1.0, -1.0};
+    const double qa = b, qb = c, qc = d;
+    const double delta = qb * qb - 4.0 * qa * qc;
+    if (fabs(qa) > CS_ROUND_EPS && delta > 0) {
+      const double ds = sqrt(delta);
+      r[0] = (-qb + ds) / (2.0 * qa);
+      r[1] = (-qb - ds) / (2.0 * qa);
+    } else if (fabs(qa) < CS_ROUND_EPS && fabs(qb) > CS_ROUND_EPS) {
+      r[0] = -qc / qb;
+    }
+    bool any = false;
+    for (int i = 0; i < 3; ++i) {
+      double t = r[i] - CS_MACH_EPS;
+      if (t < 0 || t > dt) t = -1;
+      any = any || t > CS_MACH_EPS;
+    }
+    return !any;
+  }
   b /= a;
   c /= a;
   d /= a;
@@ -502,7 +527,7 @@ __device__ bool d_is_coplanar(const Sub& S, double dt, double roots[4]) {
 // roots / *coplanar: isCoplanar's outcome for a CCD sub-test (roots -1 and
 // false when the certified screen decided it)
 __device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contrib& C, int* err,
-                          double roots[4], bool* coplanar) {
+                          double roots[4], bool* coplanar, int known) {
   double c[4][3];
   if (!P.ccd) {
 #pragma unroll
@@ -515,7 +540,10 @@ __device__ bool d_subtest(const Sub& S, bool is_pt, const NarrowParams& P, Contr
   roots[0] = roots[1] = roots[2] = -1.0;
   roots[3] = P.dt;
   *coplanar = false;
-  if (P.mixed && d_coplanar_certainly_false(S, P.dt)) return false;
+  // known: 1 = the certified screen already proved isCoplanar false, 0 = it
+  // could not, -1 = not screened yet
+  if (known == 1) return false;
+  if (known < 0 && P.mixed && d_coplanar_certainly_false(S, P.dt)) return false;
   if (!d_is_coplanar(S, P.dt, roots)) return false;
   *coplanar = true;
   for (int r = 0; r < 4; ++r) {
@@ -827,41 +855,70 @@ __device__ void collect_subtest(const Sub& S, const NarrowParams& P, int B, int 
   P.dst[slot] = (coplanar && r >= 0) ? (float)(r / P.dt) : 0.f;
 }
 
-// Learned-form statistics of a CCD sub-test (all lanes of the wavefront
-// call it): prediction vs isCoplanar, and for a sub-test with roots the
+__device__ inline int wave_sum(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return v;
+}
+
+// Learned-form statistics of a scored CCD sub-test (every lane of the
+// wavefront calls it, active = scored; one atomic per statistic and
+// wavefront, into the work-group's copy): prediction vs isCoplanar, and for a sub-test with roots the
 // bracketed Newton solve from the network's seed and from dt / 2, compared
 // with the nearest valid closed-form root (+ MACH_EPS, which isCoplanar
 // subtracts)
 __device__ void learned_stats(const NarrowParams& P, bool active, bool coplanar,
-                              const double roots[4], int64_t pi, int s, const Sub& S,
-                              int* counters) {
+                              const double roots[4], float sv, const Sub& S, int* sp) {
   bool pred = false;
+  int found = 0, its = 0, itm = 0;
+  float emax = 0.0f;
   if (active) {
-    const float sv = P.lseed[pi * SUB_GROUP + s];
     pred = signbit(sv);
     if (coplanar) {
       double a, b, c, d, rn = 0.0, rm = 0.0;
       cubic_coeffs(S.xo, S.v, &a, &b, &c, &d);
-      const int it = newton_first_root(a, b, c, d, P.dt, (double)fabsf(sv) * P.dt, &rn);
-      const int im = newton_first_root(a, b, c, d, P.dt, 0.5 * P.dt, &rm);
+      // the window isCoplanar accepts (a root r with r - MACH_EPS in [0, dt])
+      const int it = newton_first_root(a, b, c, d, CS_MACH_EPS, P.dt + CS_MACH_EPS,
+                                       (double)fabsf(sv) * P.dt, &rn);
+      const int im = newton_first_root(a, b, c, d, CS_MACH_EPS, P.dt + CS_MACH_EPS, 0.5 * P.dt, &rm);
       if (it >= 0) {
         double e = 1e300;
         for (int r = 0; r < 3; ++r)
           if (roots[r] >= 0) e = fmin(e, fabs(rn - (roots[r] + CS_MACH_EPS)));
-        atomicAdd(&counters[CNT_LN_ROOTS], 1);
-        atomicAdd(&counters[CNT_LN_ITERS], it);
-        atomicAdd(&counters[CNT_LN_ITERS_MID], im >= 0 ? im : 0);
-        if (e < 1e300) atomicMax(&counters[CNT_LN_EMAX], __float_as_int((float)(e / P.dt)));
+        found = 1;
+        its = it;
+        itm = im >= 0 ? im : 0;
+        if (e < 1e300) emax = (float)(e / P.dt);
       }
     }
   }
   const uint64_t b_sc = __ballot(active), b_ex = __ballot(active && coplanar);
   const uint64_t b_pr = __ballot(active && pred), b_ag = __ballot(active && pred == coplanar);
+  const uint64_t b_nt = __ballot(found != 0);
+  int sums[3] = {0, 0, 0};
+  float em = 0.0f;
+  if (b_nt) {
+    sums[0] = wave_sum(found);
+    sums[1] = wave_sum(its);
+    sums[2] = wave_sum(itm);
+    em = wave_max(emax);
+  }
   if ((threadIdx.x & 63) == 0 && b_sc) {
-    atomicAdd(&counters[CNT_LN_SCORED], __popcll(b_sc));
-    atomicAdd(&counters[CNT_LN_EXISTS], __popcll(b_ex));
-    atomicAdd(&counters[CNT_LN_PRED], __popcll(b_pr));
-    atomicAdd(&counters[CNT_LN_AGREE], __popcll(b_ag));
+    atomicAdd(&sp[SP_LN_SCORED], __popcll(b_sc));
+    atomicAdd(&sp[SP_LN_EXISTS], __popcll(b_ex));
+    atomicAdd(&sp[SP_LN_PRED], __popcll(b_pr));
+    atomicAdd(&sp[SP_LN_AGREE], __popcll(b_ag));
+    if (b_nt) {
+      atomicAdd(&sp[SP_LN_ROOTS], sums[0]);
+      atomicAdd(&sp[SP_LN_ITERS], sums[1]);
+      atomicAdd(&sp[SP_LN_ITERS_MID], sums[2]);
+      atomicMax(&sp[SP_LN_EMAX], __float_as_int(em));
+    }
   }
 }
 
@@ -893,6 +950,13 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
   const bool active = s < nsub;
   int q[4] = {0, 0, 0, 0};
   bool hit = false, rejected = false, coplanar = false;
+  MixedErr me{0.0f, 0.0f, 0.0f};
+  int known = -1;     // learned form: the screen's verdict for this slot
+  float lcode = 0.0f;
+  if (P.ccd && P.lseed && active) {
+    lcode = P.lseed[pi * SUB_GROUP + s];
+    known = lcode == SEED_CERTAIN ? 1 : 0;
+  }
   double roots[4] = {-1.0, -1.0, -1.0, -1.0};
   Sub S;
   Contrib C;
@@ -914,24 +978,30 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
       q[p] = vid;  // from here on: global vertex ids
     }
     if (P.mixed && !P.ccd) {
-      MixedErr me{0.0f, 0.0f, 0.0f};
       rejected = is_pt ? screen_pt(S, P.thickness, P.round_eps, P.mixed_stats, &me)
                        : screen_ee(S, P.thickness, P.mixed_stats, &me);
-      if (P.mixed_stats) {
-        atomicMax(&counters[CNT_MX_EDIST], __float_as_int(me.dist));
-        atomicMax(&counters[CNT_MX_EPARAM], __float_as_int(me.param));
-        atomicMax(&counters[CNT_MX_RATIO], __float_as_int(me.ratio));
-      }
     }
-    if (!rejected) hit = d_subtest(S, is_pt, P, C, err, roots, &coplanar);
+    if (!rejected) hit = d_subtest(S, is_pt, P, C, err, roots, &coplanar, known);
     if (P.ccd && P.ds_cap > 0) collect_subtest(S, P, B, A, s, coplanar, roots);
   }
-  if (P.ccd && P.lseed) learned_stats(P, active, coplanar, roots, pi, s, S, counters);
-  if (P.mixed && !P.ccd) {  // every lane of the wavefront takes part in the ballots
+  int* sp = P.spread + (blockIdx.x & (SPREAD_SLOTS - 1)) * SPREAD_STRIDE;
+  if (P.ccd && P.lseed) learned_stats(P, known == 0, coplanar, roots, lcode, S, sp);
+  if (P.mixed && !P.ccd) {  // every lane of the wavefront takes part
     const uint64_t rb = __ballot(active && rejected), kb = __ballot(active && !rejected);
+    float md = 0.0f, mp = 0.0f, mr = 0.0f;
+    if (P.mixed_stats) {
+      md = wave_max(me.dist);
+      mp = wave_max(me.param);
+      mr = wave_max(me.ratio);
+    }
     if (lane == 0 && (rb | kb)) {
-      atomicAdd(&counters[CNT_MX_SCREENED], __popcll(rb));
-      atomicAdd(&counters[CNT_MX_RECHECK], __popcll(kb));
+      atomicAdd(&sp[SP_MX_SCREENED], __popcll(rb));
+      atomicAdd(&sp[SP_MX_RECHECK], __popcll(kb));
+      if (P.mixed_stats) {
+        atomicMax(&sp[SP_MX_EDIST], __float_as_int(md));
+        atomicMax(&sp[SP_MX_EPARAM], __float_as_int(mp));
+        atomicMax(&sp[SP_MX_RATIO], __float_as_int(mr));
+      }
     }
   }
   // hit records: one atomic per wavefront
@@ -1007,52 +1077,153 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
                      S.hit_cap, S.hp, S.hp_cap, S.counters);
 }
 
-// Learned form: every CCD sub-test of the pair stream through the network,
-// four pairs (64 sub-test slots, the 16-lane groups of the narrow phase) per
-// tile.  Each slot keeps its seed sigmoid(s) in [0, 1] with the sign bit set
-// when the logit predicts a root; each pair gets the key that sorts it by
-// decreasing best logit (pairs past the device-side count sort last, after
-// every counted pair, since the sort is stable).
-__global__ void __launch_bounds__(MLP_THREADS)
-    k_learned_score(const int2* __restrict__ pairs, int npairs, const int* __restrict__ npairs_dev,
-                    int cap, const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
-                    const dvec4* __restrict__ av, double dt, double h,
-                    const uint16_t* __restrict__ wb, const float* __restrict__ w,
-                    float* __restrict__ seeds, uint32_t* __restrict__ keys,
-                    uint32_t* __restrict__ idx) {
-  __shared__ __align__(16) uint16_t x[MT * MF_INP];
-  __shared__ __align__(16) uint16_t h1[MT * MF_H];
-  __shared__ __align__(16) uint16_t h2[MT * MF_H];
-  __shared__ float o[MT * MF_OUT];
+// Learned form, step 1: the certified coplanarity screen of every CCD
+// sub-test of the pair stream (4 slots per thread, 16 per pair).  A slot's
+// code in seeds[]: SEED_INACTIVE (no such sub-test), SEED_CERTAIN (the
+// screen proves isCoplanar false) or 0 (to be scored; the network's output
+// replaces it).  The uncertain slots are appended to `list` (count[0]) and
+// the pairs holding one are counted (count[1]), one atomic per work-group
+// each; every pair gets the key that sorts it after all scored pairs until
+// a scored slot lowers it.
+constexpr int SCREEN_THREADS = 256, SCREEN_SLOTS = 4 * SCREEN_THREADS;
+
+__global__ void __launch_bounds__(SCREEN_THREADS)
+    k_learned_screen(const int2* __restrict__ pairs, int npairs, const int* __restrict__ npairs_dev,
+                     int cap, const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
+                     const dvec4* __restrict__ av, double dt, float* __restrict__ seeds,
+                     uint32_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                     uint32_t* __restrict__ list, int* __restrict__ count) {
+  __shared__ int wcount[SCREEN_THREADS / 64][2];
+  __shared__ int wbase[SCREEN_THREADS / 64];
   const int n = npairs_dev ? min(min(*npairs_dev, cap), npairs) : npairs;
-  const int ntile = (npairs + 3) / 4;
-  const int t = threadIdx.x;
-  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-    const int i = tile * 4 + t / SUB_GROUP, s = t % SUB_GROUP;
-    bool act = false;
-    if (t < MT) {
-      float f[MF_IN];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t base = (int64_t)blockIdx.x * SCREEN_SLOTS;
+  bool unc[4];
+  int mine = 0, mpairs = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // a pair's 16 slots are 16 consecutive lanes of one wavefront
+    const int64_t slot = base + k * SCREEN_THREADS + t;
+    const int64_t i = slot / SUB_GROUP;
+    const int s = (int)(slot % SUB_GROUP);
+    unc[k] = false;
+    if (i < npairs) {
+      float code = SEED_INACTIVE;
       if (i < n) {
         int A, B, kind, nsub, g[6];
         pair_points(pairs[i], ev, &A, &B, &kind, &nsub, g);
+        if (s == 0) {
+          keys[i] = KEY_NONE;
+          idx[i] = (uint32_t)i;
+        }
         if (s < nsub) {
-          act = true;
           int q[4];
           bool is_pt;
           subtest_points(kind, s, 1, q, &is_pt);
-          double xo[4][3], v[4][3];
+          Sub S;
 #pragma unroll
           for (int p = 0; p < 4; ++p) {
             int vid = g[0];
 #pragma unroll
-            for (int k = 1; k < 6; ++k)
-              if (q[p] == k) vid = g[k];
+            for (int m = 1; m < 6; ++m)
+              if (q[p] == m) vid = g[m];
             const dvec4 a = x_old[vid], b = av[vid];
-            xo[p][0] = a.x; xo[p][1] = a.y; xo[p][2] = a.z;
-            v[p][0] = b.x; v[p][1] = b.y; v[p][2] = b.z;
+            S.xo[p][0] = a.x; S.xo[p][1] = a.y; S.xo[p][2] = a.z;
+            S.v[p][0] = b.x; S.v[p][1] = b.y; S.v[p][2] = b.z;
           }
-          subtest_features(xo, v, dt, h, f);
+          unc[k] = !d_coplanar_certainly_false(S, dt);
+          code = unc[k] ? 0.0f : SEED_CERTAIN;
         }
+      } else if (s == 0) {
+        keys[i] = KEY_NONE;    // past the device-side count: the stable sort keeps
+        idx[i] = (uint32_t)i;  // it after every counted pair
+      }
+      seeds[slot] = code;
+    }
+    mine += unc[k];
+    // pairs with at least one slot to score (counted by their slot 0)
+    const uint64_t m = __ballot(unc[k]);
+    if ((lane & (SUB_GROUP - 1)) == 0 && ((m >> (lane & ~(SUB_GROUP - 1))) & 0xffffull)) ++mpairs;
+  }
+  // one atomic per work-group and counter
+  int wsum = mine, psum = mpairs;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    wsum += __shfl_xor(wsum, off);
+    psum += __shfl_xor(psum, off);
+  }
+  int pre = mine;  // inclusive prefix within the wavefront
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(pre, off);
+    if (lane >= off) pre += v;
+  }
+  if (lane == 0) {
+    wcount[wave][0] = wsum;
+    wcount[wave][1] = psum;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int tot = 0, ptot = 0;
+    for (int w = 0; w < SCREEN_THREADS / 64; ++w) {
+      wbase[w] = tot;
+      tot += wcount[w][0];
+      ptot += wcount[w][1];
+    }
+    const int b0 = tot ? atomicAdd(&count[0], tot) : 0;
+    if (ptot) atomicAdd(&count[1], ptot);
+    for (int w = 0; w < SCREEN_THREADS / 64; ++w) wbase[w] += b0;
+  }
+  __syncthreads();
+  int pos = wbase[wave] + pre - mine;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (unc[k]) list[pos++] = (uint32_t)(base + k * SCREEN_THREADS + t);
+}
+
+// Learned form, step 2: the network over the listed (uncertain) slots, 64
+// per tile; each slot's seed sigmoid(s) in [0, 1] with the sign bit set when
+// the logit predicts a root, and its pair's key lowered to the bucket of the
+// best score (254 - floor(254 sigmoid(logit))).
+__global__ void __launch_bounds__(MLP_THREADS)
+    k_learned_mlp(const int2* __restrict__ pairs, const int* __restrict__ count,
+                  const uint32_t* __restrict__ list, const int4* __restrict__ ev,
+                  const dvec4* __restrict__ x_old, const dvec4* __restrict__ av, double dt,
+                  double h, const uint16_t* __restrict__ wb, const float* __restrict__ w,
+                  float* __restrict__ seeds, uint32_t* __restrict__ keys) {
+  __shared__ __align__(16) uint16_t x[MT * MF_INP];
+  __shared__ __align__(16) uint16_t h1[MT * MF_H];
+  __shared__ __align__(16) uint16_t h2[MT * MF_H];
+  __shared__ float o[MT * MF_OUT];
+  const int nl = *count;
+  const int ntile = (nl + MT - 1) / MT;
+  const int t = threadIdx.x;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int e = tile * MT + t;
+    uint32_t slot = 0;
+    if (t < MT) {
+      float f[MF_IN];
+      const bool act = e < nl;
+      if (act) {
+        slot = list[e];
+        const int i = (int)(slot / SUB_GROUP), s = (int)(slot % SUB_GROUP);
+        int A, B, kind, nsub, g[6];
+        pair_points(pairs[i], ev, &A, &B, &kind, &nsub, g);
+        int q[4];
+        bool is_pt;
+        subtest_points(kind, s, 1, q, &is_pt);
+        double xo[4][3], v[4][3];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          int vid = g[0];
+#pragma unroll
+          for (int m = 1; m < 6; ++m)
+            if (q[p] == m) vid = g[m];
+          const dvec4 a = x_old[vid], b = av[vid];
+          xo[p][0] = a.x; xo[p][1] = a.y; xo[p][2] = a.z;
+          v[p][0] = b.x; v[p][1] = b.y; v[p][2] = b.z;
+        }
+        subtest_features(xo, v, dt, h, f);
       }
 #pragma unroll
       for (int k = 0; k < MF_INP; ++k)
@@ -1060,26 +1231,21 @@ __global__ void __launch_bounds__(MLP_THREADS)
     }
     __syncthreads();
     tile_forward(x, h1, h2, o, wb, w);
-    if (t < MT) {  // wavefront 0: one slot per lane
-      float z = -INFINITY;
-      if (act) {
-        z = o[t * MF_OUT];
-        const float sd = mlp_sigmoid(o[t * MF_OUT + 1]);
-        seeds[(int64_t)i * SUB_GROUP + s] = z > 0.f ? -sd : sd;
-      }
-      for (int off = SUB_GROUP / 2; off > 0; off >>= 1) z = fmaxf(z, __shfl_xor(z, off, SUB_GROUP));
-      if (s == 0 && i < npairs) {
-        uint32_t u = __float_as_uint(z);
-        u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // order-preserving bits
-        keys[i] = i < n ? ~u : 0xffffffffu;
-        idx[i] = (uint32_t)i;
-      }
+    if (t < MT && e < nl) {
+      const float z = o[t * MF_OUT];
+      const float sd = mlp_sigmoid(o[t * MF_OUT + 1]);
+      seeds[slot] = z > 0.f ? -sd : sd;
+      const float p = mlp_sigmoid(z);
+      const int bucket = 254 - (int)fminf(fmaxf(floorf(p * 254.0f), 0.0f), 254.0f);
+      atomicMin(&keys[slot / SUB_GROUP], (uint32_t)bucket);
     }
     __syncthreads();  // the tile's LDS is rewritten by the next one
   }
 }
 
-// scores the stream and sorts it; *perm = the pair positions in test order
+// Scores the stream and sorts its pairs by score bucket (one stable pass of
+// the radix sort over 8-bit keys); *perm = the pair positions in test order,
+// the pairs with a scored slot first (L.count[1] of them).
 hipError_t launch_learned_score(hipStream_t st, DevState& S, LearnedState& L, const int2* pairs,
                                 int npairs, const int* npairs_dev, int cap, double dt, double h,
                                 const uint32_t** perm) {
@@ -1087,17 +1253,16 @@ hipError_t launch_learned_score(hipStream_t st, DevState& S, LearnedState& L, co
   if (npairs <= 0) return hipSuccess;
   if (npairs > L.score_cap) {
     const int c = npairs + npairs / 4 + 1024;
-    if (L.seeds) (void)hipFree(L.seeds);
-    if (L.keys) (void)hipFree(L.keys);
-    if (L.idx) (void)hipFree(L.idx);
-    if (L.tkeys) (void)hipFree(L.tkeys);
-    if (L.tidx) (void)hipFree(L.tidx);
-    if (L.hist) (void)hipFree(L.hist);
-    if (L.part) (void)hipFree(L.part);
-    L.seeds = nullptr; L.keys = L.idx = L.tkeys = L.tidx = L.hist = L.part = nullptr;
+    void* bufs[] = {L.seeds, L.keys, L.idx, L.tkeys, L.tidx, L.hist, L.part, L.list};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    L.seeds = nullptr;
+    L.keys = L.idx = L.tkeys = L.tidx = L.hist = L.part = L.list = nullptr;
     L.score_cap = 0;
     const size_t hw = radix_hist_words(c), pw = scan_partial_words((int)hw);
-    CS_HIP_RET(hipMalloc((void**)&L.seeds, sizeof(float) * (size_t)c * SUB_GROUP));
+    const size_t slots = (size_t)(c + 64) * SUB_GROUP;  // whole screen blocks
+    CS_HIP_RET(hipMalloc((void**)&L.seeds, sizeof(float) * slots));
+    CS_HIP_RET(hipMalloc((void**)&L.list, sizeof(uint32_t) * slots));
     CS_HIP_RET(hipMalloc((void**)&L.keys, sizeof(uint32_t) * (size_t)c));
     CS_HIP_RET(hipMalloc((void**)&L.idx, sizeof(uint32_t) * (size_t)c));
     CS_HIP_RET(hipMalloc((void**)&L.tkeys, sizeof(uint32_t) * (size_t)c));
@@ -1106,13 +1271,21 @@ hipError_t launch_learned_score(hipStream_t st, DevState& S, LearnedState& L, co
     CS_HIP_RET(hipMalloc((void**)&L.part, sizeof(uint32_t) * pw));
     L.score_cap = c;
   }
-  const int ntile = (npairs + 3) / 4;
-  hipLaunchKernelGGL(k_learned_score, dim3((unsigned)std::min(ntile, 4096)), dim3(MLP_THREADS), 0,
-                     st, pairs, npairs, npairs_dev, cap, S.ev, S.x_old, S.av, dt, h, L.wb, L.w,
-                     L.seeds, L.keys, L.idx);
+  if (!L.count) CS_HIP_RET(hipMalloc((void**)&L.count, 2 * sizeof(int)));
+  CS_HIP_RET(hipMemsetAsync(L.count, 0, 2 * sizeof(int), st));
+  const int64_t nslots = (int64_t)npairs * SUB_GROUP;
+  const int nblk = (int)((nslots + SCREEN_SLOTS - 1) / SCREEN_SLOTS);
+  hipLaunchKernelGGL(k_learned_screen, dim3((unsigned)nblk), dim3(SCREEN_THREADS), 0, st, pairs,
+                     npairs, npairs_dev, cap, S.ev, S.x_old, S.av, dt, L.seeds, L.keys, L.idx,
+                     L.list, L.count);
+  CS_HIP_RET(hipGetLastError());
+  const int64_t max_tiles = (nslots + MT - 1) / MT;
+  hipLaunchKernelGGL(k_learned_mlp, dim3((unsigned)std::min<int64_t>(max_tiles, 1024)),
+                     dim3(MLP_THREADS), 0, st, pairs, L.count, L.list, S.ev, S.x_old, S.av, dt, h,
+                     L.wb, L.w, L.seeds, L.keys);
   CS_HIP_RET(hipGetLastError());
   bool in_tmp = false;
-  CS_HIP_RET(radix_sort_u32(st, L.keys, L.idx, L.tkeys, L.tidx, L.hist, L.part, npairs, 32, &in_tmp));
+  CS_HIP_RET(radix_sort_u32(st, L.keys, L.idx, L.tkeys, L.tidx, L.hist, L.part, npairs, 8, &in_tmp));
   *perm = in_tmp ? L.tidx : L.idx;
   return hipSuccess;
 }
