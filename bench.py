@@ -68,6 +68,15 @@ def parse():
     ap.add_argument("--train-stride", type=int, default=16)
     ap.add_argument("--train-samples", type=int, default=4_000_000)
     ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-scenes measurement (config 2)")
+    ap.add_argument("--batch-scenes", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=170,
+                    help="steps of the batch (scene 284 of the seeded configuration reaches the "
+                         "reference's clean_up(ERROR), a degenerate proximity contact, at step "
+                         "176 -- the oracle alone stops there too)")
+    ap.add_argument("--batch-separate-steps", type=int, default=10,
+                    help="steps of the one-handle-per-scene comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run three 256 MiB device copies (scale for HBM byte counters)")
     return ap.parse_args()
@@ -166,6 +175,62 @@ def train_network(args):
     return w, info
 
 
+def run_batch(args):
+    """Config 2: 512 uneven cloth-over-sphere scenes in one mesh, one launch
+    per kernel for all of them (cs_step_batch), against one solver handle
+    per scene stepped in turn, on the same first steps."""
+    from collision_amd import scenes as S
+    from collision_amd.solver import solver_for_scene
+    sc = S.batched_scenes(count=args.batch_scenes, steps=args.batch_steps)
+    parts = S.scene_parts(sc)
+    g = solver_for_scene(sc, pass_timing=0, log_pairs=0)
+    g.synchronize()
+    k0 = args.batch_separate_steps
+    t0 = time.perf_counter()
+    passes = syncs = 0
+    t_first = None
+    for k in range(args.batch_steps):
+        rep, _ = g.stepBatch()
+        passes += rep.n_passes
+        syncs += rep.host_syncs
+        if k + 1 == k0:
+            g.synchronize()
+            t_first = time.perf_counter() - t0
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.close()
+    hs = [solver_for_scene(p, pass_timing=0, log_pairs=0) for p in parts]
+    for h in hs:
+        h.synchronize()
+    t1 = time.perf_counter()
+    sep_syncs = 0
+    for _ in range(k0):
+        for h in hs:
+            sep_syncs += h.step().host_syncs
+    for h in hs:
+        h.synchronize()
+    sep = time.perf_counter() - t1
+    for h in hs:
+        h.close()
+    res = {"scenes": args.batch_scenes, "vertices": sc.nv, "triangles": sc.nt,
+           "steps": args.batch_steps, "steps_per_s": args.batch_steps / el,
+           "ms_per_step": 1e3 * el / args.batch_steps,
+           "passes_per_step": passes / args.batch_steps,
+           "host_syncs_per_step": syncs / args.batch_steps,
+           "first_steps": k0,
+           "first_steps_per_s": k0 / t_first if t_first else None,
+           "separate_handles_first_steps_per_s": k0 / sep,
+           "separate_host_syncs_per_step": sep_syncs / k0,
+           "batched_over_separate": (sep / t_first) if t_first else None,
+           "note": "batched: one mesh, one launch per kernel for every scene; separate: one "
+                   "solver handle per scene, stepped in turn; both on steps 1.."
+                   f"{k0} (no contact yet in that window)"}
+    print(f"[bench] batch {res['steps_per_s']:.2f} steps/s; first {k0}: batched "
+          f"{res['first_steps_per_s']:.2f} vs separate {res['separate_handles_first_steps_per_s']:.3f}",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def state_digest(solver):
     import hashlib
     st = solver.getState()
@@ -257,6 +322,7 @@ def main():
     steps_per_s = args.steps / wall
 
     forms = None if args.no_forms else run_forms(sc, weights)
+    batch = None if args.no_batch else run_batch(args)
     config = None
     if forms is not None and args.form in forms:
         config = {"steps_per_s": forms[args.form]["steps_per_s"],
@@ -349,6 +415,7 @@ def main():
         "forms_identical_state": (len({f["state_digest"] for f in forms.values()}) == 1
                                   if forms else None),
         "training": training,
+        "batch": batch,
         "pairs_per_s": tested / t_narrow_all if t_narrow_all > 0 else None,
         "pairs_per_s_note": "element pairs evaluated by the narrow-phase kernel per second of "
                             "its run time (full and incremental passes)",

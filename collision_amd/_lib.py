@@ -48,7 +48,8 @@ class MeshView(C.Structure):
                 ("tri_len0", C.POINTER(C.c_double)), ("bonds", C.POINTER(C.c_int32)),
                 ("bond_len0", C.POINTER(C.c_double)), ("is_fixed", C.POINTER(C.c_uint8)),
                 ("nbody", C.c_int32), ("vert_body", C.POINTER(C.c_int32)),
-                ("body_mass", C.POINTER(C.c_double)), ("body_com", C.POINTER(C.c_double))]
+                ("body_mass", C.POINTER(C.c_double)), ("body_com", C.POINTER(C.c_double)),
+                ("nscene", C.c_int32), ("elem_scene", C.POINTER(C.c_int32))]
 
 
 class PassInfo(C.Structure):
@@ -73,6 +74,15 @@ class StepReport(C.Structure):
                 ("host_syncs", C.c_int64), ("mx_screened", C.c_int64), ("mx_rechecked", C.c_int64),
                 ("mx_err_dist", C.c_double), ("mx_err_param", C.c_double),
                 ("mx_bound_ratio", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class SceneReport(C.Structure):
+    _fields_ = [("n_passes", C.c_int32), ("n_ccd_passes", C.c_int32),
+                ("n_zone_passes", C.c_int32), ("has_collision", C.c_int32),
+                ("zone_cap", C.c_int32), ("pad", C.c_int32), ("hit_pairs", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -124,6 +134,8 @@ def lib(path: str = LIB_PATH):
         "cs_predict": ([vp], C.c_int),
         "cs_resolve": ([vp, P(StepReport)], C.c_int),
         "cs_step": ([vp, P(StepReport)], C.c_int),
+        "cs_resolve_batch": ([vp, P(StepReport), P(SceneReport), i32], C.c_int),
+        "cs_step_batch": ([vp, P(StepReport), P(SceneReport), i32], C.c_int),
         "cs_has_collision": ([vp], C.c_int),
         "cs_synchronize": ([vp], C.c_int),
         "cs_pass_count": ([vp], i32),

@@ -149,7 +149,7 @@ void free_dev(DevState& S) {
   dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
   dfree(S.zkey); dfree(S.zval); dfree(S.zval_tmp);
   dfree(S.zhead); dfree(S.zend);
-  dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters); dfree(S.spread);
+  dfree(S.lvl_ptr); dfree(S.plog); dfree(S.counters); dfree(S.spread); dfree(S.scn); dfree(S.vscene);
   dfree(S.sslot); dfree(S.spre); dfree(S.slast);
   dfree(S.lvl_active);
   S = DevState();
@@ -586,6 +586,7 @@ int run_pass(cs_solver* h, int kind, int index, int pass_id, bool* any_hit) {
                                               : ~0ull;
   }
   mark_pass(h, &T.e2, 2);
+  launch_scene_pass(h->st, S, kind, index, nhp, P.max_ccd_passes);  // batched scenes
   HIPCHK(h, apply_hits(h->st, S, nhits, vcount_cur, P.record_impulses != 0 && S.pimp));
   HIPCHK(h, apply_bodies(h->st, S, P.mass, P.dt));  // RG impulses, body zones (updateAverageVelocity)
   int nzone = 0;
@@ -807,6 +808,46 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     }
   }
   auto movable = [&](int v) { return vbody[v] >= 0; };
+  // batched scenes: the scene of an element goes to the upper half of its
+  // surface field, the surface's id within its scene to the lower half
+  const int nscene = m->nscene;
+  if (nscene < 0 || (nscene > 1 && !m->elem_scene))
+    return fail(h, CS_ERR_ARG, "invalid batched-scene arrays");
+  if (nscene > 32767) return fail(h, CS_ERR_UNSUPPORTED, "more than 32767 scenes");
+  if (nscene > 1 && nbody > 0)
+    return fail(h, CS_ERR_UNSUPPORTED, "batched scenes with movable bodies");
+  std::vector<int32_t> vscene;
+  std::vector<std::vector<std::pair<int, int>>> scene_surf;
+  if (nscene > 1) {
+    vscene.assign((size_t)nv, -1);
+    scene_surf.resize((size_t)nscene);
+    for (int e = 0; e < ne; ++e) {
+      const int sc = m->elem_scene[e];
+      if (sc < 0 || sc >= nscene) return fail(h, CS_ERR_ARG, "element scene id out of range");
+      const int np = e < nt ? 3 : 2;
+      for (int j = 0; j < np; ++j) {
+        const int v = e < nt ? m->tris[3 * e + j] : m->bonds[2 * (e - nt) + j];
+        if (v < 0 || v >= nv) return fail(h, CS_ERR_ARG, "element vertex id out of range");
+        if (vscene[v] >= 0 && vscene[v] != sc)
+          return fail(h, CS_ERR_ARG, "elements of different scenes share a vertex");
+        vscene[v] = sc;
+      }
+    }
+  }
+  auto surf_field = [&](int e, int user_surf) -> int {
+    if (nscene <= 1) return user_surf;
+    const int sc = m->elem_scene[e];
+    if (e >= nt) return (sc << 16) | 0xffff;
+    auto& lst = scene_surf[(size_t)sc];
+    int local = -1;
+    for (auto& pr : lst)
+      if (pr.first == user_surf) local = pr.second;
+    if (local < 0) {
+      local = (int)lst.size();
+      lst.push_back({user_surf, local});
+    }
+    return (sc << 16) | local;
+  };
   for (int e = 0; e < nt; ++e) {
     int v[3];
     for (int j = 0; j < 3; ++j) {
@@ -816,7 +857,9 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     }
     if (vbody[v[1]] != vbody[v[0]] || vbody[v[2]] != vbody[v[0]])
       return fail(h, CS_ERR_ARG, "a triangle mixes vertices of different bodies");
-    ev[e] = make_int4(v[0], v[1], v[2], m->tri_surf ? m->tri_surf[e] : 0);
+    ev[e] = make_int4(v[0], v[1], v[2], surf_field(e, m->tri_surf ? m->tri_surf[e] : 0));
+    if (nscene > 1 && scene_surf[(size_t)m->elem_scene[e]].size() >= 0xffff)
+      return fail(h, CS_ERR_UNSUPPORTED, "more than 65534 surfaces in one scene");
     // isRigidBody(hse): a static or movable point (dcollid.cpp:1091-1094)
     const bool rigid = fixed[v[0]] || fixed[v[1]] || fixed[v[2]] || movable(v[0]);
     ef[e] = EF_TRI | (rigid ? EF_RIGID : 0);
@@ -830,7 +873,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
       return fail(h, CS_ERR_ARG, "bond vertex id out of range");
     used[v0] = used[v1] = 1;
     if (movable(v0) || movable(v1)) return fail(h, CS_ERR_ARG, "a bond holds a movable vertex");
-    ev[e] = make_int4(v0, v1, -1, -1);
+    ev[e] = make_int4(v0, v1, -1, surf_field(e, -1));
     ef[e] = (fixed[v0] || fixed[v1]) ? EF_RIGID : 0;
     l0[e] = dvec4{m->bond_len0 ? m->bond_len0[b] : 0.0, 0.0, 0.0, 0.0};
   }
@@ -988,6 +1031,13 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.plog, S.plog_cap));
   HIPCHK(h, dalloc(&S.counters, COUNTER_INTS));
   HIPCHK(h, dalloc(&S.spread, SPREAD_SLOTS * SPREAD_STRIDE));
+  S.nscene = nscene > 1 ? nscene : 0;
+  if (S.nscene > 1) {
+    HIPCHK(h, dalloc(&S.scn, (size_t)SCN_FIELDS * S.nscene));
+    HIPCHK(h, hipMemset(S.scn, 0, sizeof(int32_t) * SCN_FIELDS * S.nscene));
+    HIPCHK(h, dalloc(&S.vscene, (size_t)nv));
+    HIPCHK(h, hipMemcpy(S.vscene, vscene.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice));
+  }
   HIPCHK(h, alloc_sort_hist(S));
   HIPCHK(h, hipMemcpy(S.ev, ev.data(), sizeof(int4) * ne, hipMemcpyHostToDevice));
   HIPCHK(h, hipMemcpy(S.eflags, ef.data(), ne, hipMemcpyHostToDevice));
@@ -1152,6 +1202,7 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   if (spread_stats)
     HIPCHK(h, hipMemsetAsync(S.spread, 0, sizeof(int) * SPREAD_SLOTS * SPREAD_STRIDE, h->st));
   launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
+  launch_scene_reset(h->st, S);                   // batched scenes: every scene's pass loop
   if (P.record_impulses) {  // the impulses of this resolve's last pass with a hit
     if (!S.pimp) {
       HIPCHK(h, dalloc(&S.pimp, S.nv));
@@ -1291,6 +1342,38 @@ int cs_step(cs_solver* h, cs_step_report* report) {
   int rc = cs_predict(h);
   if (rc) return rc;
   return cs_resolve(h, report);
+}
+
+int cs_resolve_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
+                     int32_t nscene) {
+  if (!h) return CS_ERR_ARG;
+  if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
+  if (h->S.nscene <= 1 || nscene != h->S.nscene || !per_scene)
+    return fail(h, CS_ERR_ARG, "nscene must equal the assembled batch's scene count");
+  const int rc = cs_resolve(h, report);
+  if (rc != CS_OK && rc != CS_WARN_ZONE_CAP) return rc;
+  std::vector<int32_t> f((size_t)SCN_FIELDS * nscene);
+  HIPCHK(h, hipMemcpyAsync(f.data(), h->S.scn, sizeof(int32_t) * f.size(), hipMemcpyDeviceToHost,
+                           h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  for (int s = 0; s < nscene; ++s) {
+    cs_scene_report& r = per_scene[s];
+    r = cs_scene_report{};
+    r.n_ccd_passes = f[(size_t)SCN_CCD * nscene + s];
+    r.n_zone_passes = f[(size_t)SCN_ZONE * nscene + s];
+    r.n_passes = 1 + r.n_ccd_passes + r.n_zone_passes;
+    r.has_collision = f[(size_t)SCN_HASCOL * nscene + s];
+    r.zone_cap = f[(size_t)SCN_PHASE * nscene + s] == 2;
+    r.hit_pairs = f[(size_t)SCN_HITPAIRS * nscene + s];
+  }
+  return rc;
+}
+
+int cs_step_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
+                  int32_t nscene) {
+  int rc = cs_predict(h);
+  if (rc) return rc;
+  return cs_resolve_batch(h, report, per_scene, nscene);
 }
 
 int cs_has_collision(cs_solver* h) { return h ? h->has_collision : 0; }

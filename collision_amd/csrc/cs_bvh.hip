@@ -381,6 +381,9 @@ __device__ inline int pair_reject_kind(int4 ta, int4 tb, uint8_t fa) {
 __device__ inline bool pair_rejected(int4 ta, int4 tb, uint8_t fa) {
   return pair_reject_kind(ta, tb, fa) != 0;
 }
+// batched scenes: an element's scene is held in the upper half of its
+// surface field (the lower half is the surface's id within the scene)
+__device__ inline int scene_of(const int4& t) { return t.w >> 16; }
 
 constexpr int TRAV_THREADS = 256;
 constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flush
@@ -413,7 +416,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
                                                            int* __restrict__ pair_count,
                                                            unsigned long long* __restrict__ cand_count,
                                                            unsigned long long* __restrict__ pcand_count,
-                                                           int* __restrict__ next) {
+                                                           int* __restrict__ next, int scenes) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
   const int t = threadIdx.x, lane = t & 63;
   const uint64_t below = (1ull << lane) - 1ull;
@@ -508,6 +511,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
       // issued with the box's, not after the overlap test)
       asm volatile("" ::"v"(ot.x), "v"(ot.y), "v"(ot.z), "v"(ot.w), "v"((int)of));
       if (!f64_overlap(ob, mb)) continue;
+      if (scenes && scene_of(ot) != scene_of(mt)) continue;  // batched scenes never pair up
       ++cand;
       const int rk = (e > e2) ? pair_reject_kind(mt, ot, mf) : pair_reject_kind(ot, mt, of);
       if (rk) {
@@ -774,6 +778,7 @@ __global__ void __launch_bounds__(64) k_query_ranges(
     const dbox* __restrict__ ebox, const int4* __restrict__ ev,
     const uint8_t* __restrict__ eflags, const uint8_t* __restrict__ dirty_e,
     const uint8_t* __restrict__ dpre, const int32_t* __restrict__ rlist, int npre, int emit,
+    int scenes,
     int2* __restrict__ qpairs, int qcap, int* __restrict__ counters) {
   __shared__ int32_t fr[2][QW_FCAP];
   __shared__ int2 lv[QW_LCAP];  // (pre-order index, element)
@@ -873,6 +878,7 @@ __global__ void __launch_bounds__(64) k_query_ranges(
     int nbuf = 0;
     // a box-overlapping pair of query element qel and element e2
     auto take = [&](int qel, const int4& qtop, uint8_t qfl, int e2, const int4& ot, uint8_t of) {
+      if (scenes && scene_of(ot) != scene_of(qtop)) return;  // batched scenes never pair up
       ++cand;
       if (!emit) return;
       const bool rej = (qel > e2) ? pair_rejected(qtop, ot, qfl) : pair_rejected(ot, qtop, of);
@@ -1099,7 +1105,7 @@ void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
   const int npre = 2 * S.ne - 1;
   hipLaunchKernelGGL(k_query_ranges, dim3(std::min((npre + 63) / 64, 4096)), dim3(64), 0, st,
                      S.nodes, S.nright, S.ebox, S.ev, S.eflags, S.dirty_e, S.dpre, S.rlist, npre,
-                     emit, S.qpairs, S.qpair_cap, S.counters);
+                     emit, S.nscene > 1, S.qpairs, S.qpair_cap, S.counters);
 }
 
 void launch_carry_pairs_from(hipStream_t st, DevState& S, const int2* src, int n) {
@@ -1200,12 +1206,12 @@ void launch_traverse(hipStream_t st, DevState& S, int2* out, int cap, const dbox
     hipLaunchKernelGGL(k_traverse<true>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
                        S.nodes, S.lpre, S.ebox, pbox, S.ev, S.eflags, n, out, cap,
                        S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
-                       (unsigned long long*)(S.counters + CNT_PCAND64), S.tnext);
+                       (unsigned long long*)(S.counters + CNT_PCAND64), S.tnext, S.nscene > 1);
   else
     hipLaunchKernelGGL(k_traverse<false>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
                        S.nodes, S.lpre, S.ebox, nullptr, S.ev, S.eflags, n, out, cap,
                        S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
-                       nullptr, S.tnext);
+                       nullptr, S.tnext, S.nscene > 1);
 }
 
 void launch_split_seed(hipStream_t st, DevState& S, int n) {

@@ -233,7 +233,27 @@ struct DevState {
   // counters / flags
   int* counters = nullptr;
   int* spread = nullptr;  // [SPREAD_SLOTS][SPREAD_STRIDE] statistics (SP_*)
+  // batched scenes (nscene > 1): per-scene pass state, SCN_FIELDS x nscene
+  int nscene = 0;
+  int32_t* scn = nullptr;
+  int32_t* vscene = nullptr;  // scene of every vertex (-1: none)
 };
+
+// per-scene pass state of a batched resolve (rows of DevState::scn)
+enum {
+  SCN_PHASE = 0,   // 0 before the CCD loop, 1 in it, 2 in the impact-zone loop, 3 done
+  SCN_CCD,         // CCD passes the scene runs
+  SCN_ZONE,        // impact-zone passes it runs
+  SCN_HASCOL,      // hit in its first CCD pass
+  SCN_HITPAIRS,    // hit pairs over the passes it runs
+  SCN_PASS_HIT,    // this pass: hit pairs of the scene (reset after each pass)
+  SCN_ACTIVE,      // this pass belongs to the scene's own pass loop
+  SCN_FIELDS
+};
+void launch_scene_reset(hipStream_t st, DevState& S);
+// after a pass's narrow phase: scene of each of its nhp hit pairs, then the
+// per-scene state machine of the reference's pass loop
+void launch_scene_pass(hipStream_t st, DevState& S, int kind, int index, int nhp, int max_ccd);
 
 // The time-of-impact proposer of a handle: FP32 master weights, Adam state,
 // BF16 matrix-core copies, the scoring and sorting buffers of the learned

@@ -284,7 +284,8 @@ __global__ void __launch_bounds__(ZONE_LANES)
                   const int32_t* __restrict__ bptr, int nbz,
                   const dvec4* __restrict__ x_old, dvec4* __restrict__ av,
                   const uint8_t* __restrict__ fixed, double m, double dt,
-                  int* __restrict__ counters) {
+                  int* __restrict__ counters, const int32_t* __restrict__ vscene,
+                  const int32_t* __restrict__ scene_active) {
   __shared__ double part[ZONE_LANES][9];
   __shared__ double cpos[ZONE_CACHE][3], cvel[ZONE_CACHE][3];
   __shared__ int cid[ZONE_CACHE];  // vertex id, -1 - id when fixed
@@ -295,6 +296,9 @@ __global__ void __launch_bounds__(ZONE_LANES)
     const int b = bptr ? bptr[z] : zhead[z];
     const int e = bptr ? bptr[z + 1] : zend[zkey[b]];
     const int n = e - b;
+    // batched scenes: only the zones of scenes still in their impact-zone
+    // loop move (a repeated rigid update is not bitwise idempotent)
+    if (vscene && !scene_active[vscene[zval[b]]]) continue;
     // member j (b <= j < e), from the cache when it holds it
     auto member = [&](int j) -> ZoneMember {
       ZoneMember r;
@@ -447,7 +451,8 @@ hipError_t update_zones(hipStream_t st, DevState& S, double m, double dt, int nl
                      S.zend, S.counters);
   hipLaunchKernelGGL(k_zone_update, dim3(std::min(nlist, 2048)), dim3(ZONE_LANES), 0, st, S.zkey,
                      zv, S.zhead, S.zend, (const int32_t*)nullptr, 0, S.x_old, S.av, S.fixed, m, dt,
-                     S.counters);
+                     S.counters, S.nscene > 1 ? S.vscene : nullptr,
+                     S.nscene > 1 ? S.scn + SCN_ACTIVE * S.nscene : nullptr);
   return hipGetLastError();
 }
 
@@ -456,7 +461,7 @@ hipError_t update_body_zones(hipStream_t st, DevState& S, double m, double dt) {
   hipLaunchKernelGGL(k_zone_update, dim3(std::min(S.nbody, 2048)), dim3(ZONE_LANES), 0, st,
                      (const uint32_t*)nullptr, S.bmem, (const int32_t*)nullptr,
                      (const int32_t*)nullptr, S.bptr, S.nbody, S.x_old, S.av, S.fixed, m, dt,
-                     S.counters);
+                     S.counters, (const int32_t*)nullptr, (const int32_t*)nullptr);
   return hipGetLastError();
 }
 

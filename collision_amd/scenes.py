@@ -149,6 +149,7 @@ class Scene:
     domain_lo: tuple = (-10.0, -10.0, -10.0)
     domain_hi: tuple = (10.0, 10.0, 10.0)
     scene_of_vertex: Optional[np.ndarray] = None   # batched scenes only
+    elem_scene: Optional[np.ndarray] = None        # batched scenes: scene of every element
     # movable rigid bodies (MOVABLE_BODY_BOUNDARY surfaces): body of every
     # vertex (-1: none), total mass and centre of mass of every body
     vert_body: Optional[np.ndarray] = None
@@ -448,7 +449,7 @@ def batched_scenes(count=512, nmin=16, nmax=128, seed=None, spacing=2.0,
     lo, hi = det_log(np.array([float(nmin)]))[0], det_log(np.array([float(nmax)]))[0]
     b = _Builder()
     cols = int(np.ceil(np.sqrt(count)))
-    sizes, owner = [], []
+    sizes, owner, tri_owner = [], [], []
     for s in range(count):
         un, ur, ul = u[3 * s:3 * s + 3]
         n = int(np.floor(_det_exp(lo + (hi - lo) * un) + 0.5))
@@ -464,11 +465,37 @@ def batched_scenes(count=512, nmin=16, nmax=128, seed=None, spacing=2.0,
         sx, st = icosphere(level, r, center=(ox, oy, 0.0))
         b.add_surface(sx, np.zeros_like(sx), st, fixed=True)
         owner += [s] * (len(x) + len(sx))
+        tri_owner += [s] * (len(t) + len(st))
         sizes.append((n, level, r))
     sc = b.build(name, thickness=thickness, dt=dt, steps=steps)
     sc.scene_of_vertex = np.array(owner, np.int32)
+    sc.elem_scene = np.array(tri_owner, np.int32)
     sc.meta["scenes"] = sizes
     return sc
+
+
+def scene_parts(sc: Scene) -> List[Scene]:
+    """The scenes of a batch (`elem_scene`, triangles only) as separate
+    scenes, vertices and elements renumbered in their batch order."""
+    assert sc.elem_scene is not None and sc.nb == 0
+    out = []
+    for s in range(int(sc.elem_scene.max()) + 1):
+        tsel = np.flatnonzero(sc.elem_scene == s)
+        vsel = np.flatnonzero(sc.scene_of_vertex == s)
+        remap = np.full(sc.nv, -1, np.int64)
+        remap[vsel] = np.arange(len(vsel))
+        surf = sc.tri_surf[tsel]
+        _, surf_local = np.unique(surf, return_inverse=True)
+        out.append(Scene(name=f"{sc.name}_{s}", x=sc.x[vsel].copy(), vel=sc.vel[vsel].copy(),
+                         fixed=sc.fixed[vsel].copy(),
+                         tris=np.ascontiguousarray(remap[sc.tris[tsel]], np.int32),
+                         tri_surf=np.ascontiguousarray(surf_local, np.int32),
+                         tri_len0=sc.tri_len0[tsel].copy(), bonds=np.zeros((0, 2), np.int32),
+                         bond_len0=np.zeros(0), thickness=sc.thickness, dt=sc.dt,
+                         steps=sc.steps, friction=sc.friction, round_eps=sc.round_eps,
+                         spring_k=sc.spring_k, mass=sc.mass, restitution=sc.restitution,
+                         domain_lo=sc.domain_lo, domain_hi=sc.domain_hi))
+    return out
 
 
 def _det_exp(x: float) -> float:

@@ -107,7 +107,7 @@ class CollisionSolver3d:
 
     # -- topology and state --
     def assemble(self, nv, tris, tri_surf, tri_len0, bonds, bond_len0, is_fixed,
-                 vert_body=None, body_mass=None, body_com=None):
+                 vert_body=None, body_mass=None, body_com=None, elem_scene=None):
         arrs = dict(tris=np.ascontiguousarray(tris, np.int32).reshape(-1, 3),
                     tri_surf=np.ascontiguousarray(tri_surf, np.int32),
                     tri_len0=np.ascontiguousarray(tri_len0, np.float64).reshape(-1, 3),
@@ -119,21 +119,28 @@ class CollisionSolver3d:
             arrs.update(vert_body=np.ascontiguousarray(vert_body, np.int32),
                         body_mass=np.ascontiguousarray(body_mass, np.float64),
                         body_com=np.ascontiguousarray(body_com, np.float64).reshape(-1, 3))
+        nscene = 0
+        if elem_scene is not None:
+            arrs["elem_scene"] = np.ascontiguousarray(elem_scene, np.int32)
+            nscene = int(arrs["elem_scene"].max()) + 1 if arrs["elem_scene"].size else 0
         m = MeshView(nv, arrs["tris"].shape[0], arrs["bonds"].shape[0],
                      _p(arrs["tris"], C.c_int32), _p(arrs["tri_surf"], C.c_int32),
                      _p(arrs["tri_len0"], C.c_double), _p(arrs["bonds"], C.c_int32),
                      _p(arrs["bond_len0"], C.c_double), _p(arrs["is_fixed"], C.c_uint8),
                      nbody, _p(arrs.get("vert_body"), C.c_int32),
-                     _p(arrs.get("body_mass"), C.c_double), _p(arrs.get("body_com"), C.c_double))
+                     _p(arrs.get("body_mass"), C.c_double), _p(arrs.get("body_com"), C.c_double),
+                     nscene, _p(arrs.get("elem_scene"), C.c_int32))
         self._check(self.L.cs_assemble(self.h, C.byref(m)))
         self.nv = nv
         self.nbody = nbody
+        self.nscene = nscene
 
     def assembleFromMesh(self, scene):
         """assembleFromInterface over a flat scene (collision_amd.scenes.Scene)."""
         self.assemble(scene.nv, scene.tris, scene.tri_surf, scene.tri_len0, scene.bonds,
                       scene.bond_len0, scene.fixed, getattr(scene, "vert_body", None),
-                      getattr(scene, "body_mass", None), getattr(scene, "body_com", None))
+                      getattr(scene, "body_mass", None), getattr(scene, "body_com", None),
+                      getattr(scene, "elem_scene", None))
 
     def setState(self, x, vel, has_collsn=None):
         x = np.ascontiguousarray(x, np.float64)
@@ -169,6 +176,20 @@ class CollisionSolver3d:
     def step(self) -> StepReport:
         self.predict()
         return self.resolveCollision()
+
+    # -- batched scenes (no counterpart in the reference) --
+    def resolveBatch(self):
+        """cs_resolve_batch: (batch report, list of per-scene report dicts)"""
+        rep = StepReport()
+        per = (_lib.SceneReport * self.nscene)()
+        self._check(self.L.cs_resolve_batch(self.h, C.byref(rep), per, self.nscene),
+                    allow_warn=True)
+        self.last_report = rep
+        return rep, [r.as_dict() for r in per]
+
+    def stepBatch(self):
+        self.predict()
+        return self.resolveBatch()
 
     def hasCollision(self) -> bool:
         return bool(self.L.cs_has_collision(self.h))

@@ -102,6 +102,15 @@ typedef struct cs_mesh_view {
     const int32_t *vert_body; /* [nv] body of a movable vertex, -1 otherwise      */
     const double *body_mass;  /* [nbody] total_mass (dcollid3d.cpp:968-969)     */
     const double *body_com;   /* [nbody][3] center_of_mass (dcollid.cpp:641-650) */
+    /* batched scenes (no counterpart in the reference, which runs one scene
+     * per solver): nscene > 1 independent scenes in one mesh, elem_scene[e]
+     * the scene of element e (triangles then bonds).  Elements of different
+     * scenes share no vertex and are never paired, so each scene runs as it
+     * would alone; cs_resolve_batch reports every scene's own pass loop.
+     * Scenes laid out apart in space keep the broad phase from meeting
+     * other scenes' boxes at all.  Leave nscene = 0 for one scene. */
+    int32_t nscene;
+    const int32_t *elem_scene;
 } cs_mesh_view;
 
 typedef struct cs_pass_info {
@@ -195,6 +204,25 @@ int cs_predict(cs_solver *h);
 int cs_resolve(cs_solver *h, cs_step_report *report);
 /* cs_predict + cs_resolve */
 int cs_step(cs_solver *h, cs_step_report *report);
+
+/* One scene's share of a batched resolve: the pass loop resolveCollision
+ * runs for that scene alone (dcollid.cpp:317-362) */
+typedef struct cs_scene_report {
+    int32_t n_passes;        /* 1 + n_ccd_passes + n_zone_passes */
+    int32_t n_ccd_passes, n_zone_passes;
+    int32_t has_collision;   /* hit in the scene's first CCD pass */
+    int32_t zone_cap;        /* the impact-zone loop stopped at max_zone_passes */
+    int32_t pad;
+    int64_t hit_pairs;       /* hit pairs over the scene's passes */
+} cs_scene_report;
+/* cs_resolve over all scenes of a batched mesh (one launch per kernel for
+ * every scene), then every scene's report: per_scene[nscene], nscene as
+ * assembled.  report (may be NULL) covers the whole batch. */
+int cs_resolve_batch(cs_solver *h, cs_step_report *report, cs_scene_report *per_scene,
+                     int32_t nscene);
+/* cs_predict + cs_resolve_batch */
+int cs_step_batch(cs_solver *h, cs_step_report *report, cs_scene_report *per_scene,
+                  int32_t nscene);
 /* CollisionSolver::hasCollision(): 1 if the first CCD pass of the last
  * resolve found a collision */
 int cs_has_collision(cs_solver *h);
@@ -268,7 +296,10 @@ int64_t cs_learned_dataset_size(cs_solver *h);
 /* copy the dataset out: x[n][24] (float), exists[n], toi[n] */
 int cs_learned_dataset(cs_solver *h, float *x, float *exists, float *toi, int64_t cap);
 /* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over shuffled batches of the
- * collected dataset, all on the device; loss[i] = mean loss of epoch i */
+ * collected dataset, all on the device; loss[i] = mean loss of epoch i.
+ * The classes are balanced by repetition: each sample with a root appears
+ * round(#without / #with) times per epoch (1 to 64).  The same dataset,
+ * weights and seed give the same weights bit for bit. */
 int cs_learned_train(cs_solver *h, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                      float *loss);
 /* statistics of the learned form since the previous call (which resets them) */
