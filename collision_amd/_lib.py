@@ -39,7 +39,7 @@ class Params(C.Structure):
                 ("log_pairs", C.c_int32), ("device", C.c_int32),
                 ("force_full_passes", C.c_int32), ("pass_timing", C.c_int32),
                 ("record_impulses", C.c_int32), ("precision", C.c_int32),
-                ("learned", C.c_int32), ("precision_stats", C.c_int32), ("pad1", C.c_int32)]
+                ("learned", C.c_int32), ("precision_stats", C.c_int32), ("graph_step", C.c_int32)]
 
 
 class MeshView(C.Structure):

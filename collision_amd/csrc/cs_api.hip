@@ -97,6 +97,21 @@ struct cs_solver {
   int64_t syncs = 0;  // host synchronisations of the current resolve
   LearnedState L;           // time-of-impact proposer (learned form)
   cs_learned_stats lstats{};  // its statistics since the last cs_learned_get_stats
+  // graph_step: the captured predictor + resolve prologue and what it was
+  // captured for (a change of any of these captures it again)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  struct GraphKey {
+    int64_t assemble_gen = -1;
+    double dt = 0.0;
+    int record = 0, stats = 0;
+    void* pimp = nullptr;
+    bool operator==(const GraphKey& o) const {
+      return assemble_gen == o.assemble_gen && dt == o.dt && record == o.record &&
+             stats == o.stats && pimp == o.pimp;
+    }
+  } graph_key;
+  int64_t assemble_gen = 0;  // bumped by every cs_assemble / cs_load
 };
 
 namespace {
@@ -746,6 +761,8 @@ int cs_destroy(cs_solver* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   free_dev(h->S);
   learned_free(h->L);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   for (auto e : h->events) (void)hipEventDestroy(e);
   if (h->h_counters) (void)hipHostFree(h->h_counters);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -1113,6 +1130,7 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
     HIPCHK(h, hipMemset(S.rg_num, 0, sizeof(int32_t) * nbody));
   }
   h->assembled = true;
+  h->assemble_gen++;  // buffers changed: a captured step graph is stale
   h->state_set = false;
   return CS_OK;
 }
@@ -1176,7 +1194,58 @@ int cs_predict(cs_solver* h) {
   return CS_OK;
 }
 
-int cs_resolve(cs_solver* h, cs_step_report* report) {
+namespace {
+
+// The launches of a resolve before its first host synchronisation (kernels
+// only, so that they can be captured in a graph): the dirty-range marks of
+// the last pass are cleared while their count is still in the counter block,
+// then the counters, the union-find (makeSet, dcollid.cpp:1015-1031), the
+// batched scenes' pass state, the recorded impulses, the zone marks and
+// computeAverageVelocity.
+void resolve_prologue(cs_solver* h, bool stats) {
+  DevState& S = h->S;
+  const cs_params& P = h->P;
+  if (S.ne >= 2) launch_clear_ranges(h->st, S);
+  launch_fill_u32(h->st, S.counters, COUNTER_INTS, 0u);
+  if (stats) launch_fill_u32(h->st, S.spread, (int64_t)SPREAD_SLOTS * SPREAD_STRIDE, 0u);
+  launch_uf_init(h->st, S);
+  launch_scene_reset(h->st, S);
+  if (P.record_impulses && S.pimp) {  // the impulses of this resolve's last pass with a hit
+    launch_fill_u32(h->st, S.pimp, (int64_t)S.nv * (int64_t)(sizeof(dvec4) / 4), 0u);
+    launch_fill_u32(h->st, S.pnum, S.nv, 0u);
+  }
+  launch_fill_u32(h->st, S.zmark, S.nv, 0u);
+  launch_average_velocity(h->st, S, P.dt);
+}
+
+// the predictor and the resolve prologue as one graph, captured again when
+// what its launches depend on has changed
+int ensure_step_graph(cs_solver* h, bool stats) {
+  cs_solver::GraphKey key;
+  key.assemble_gen = h->assemble_gen;
+  key.dt = h->P.dt;
+  key.record = h->P.record_impulses != 0;
+  key.stats = stats;
+  key.pimp = h->S.pimp;
+  if (h->graph_exec && key == h->graph_key) return CS_OK;
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  launch_predict(h->st, h->S, h->P.dt);
+  resolve_prologue(h, stats);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(h->st, &g);
+  if (e != hipSuccess) return fail(h, CS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(h, hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+  h->graph_key = key;
+  return CS_OK;
+}
+
+// with_predict_graph: the predictor runs first, inside the captured graph
+int resolve_impl(cs_solver* h, cs_step_report* report, bool with_predict_graph) {
   int rc = check_ready(h);
   if (rc) return rc;
   if (h->P.learned && !h->L.ready)
@@ -1193,26 +1262,20 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   h->prev = PrevPass();
   hipEvent_t t0, t_strain0, t_end;
   std::vector<hipEvent_t> strain_marks;
-  mark(h, &t0);
-  // the dirty-range marks of the last pass are cleared while their count is
-  // still in the counter block
-  if (S.ne >= 2) launch_clear_ranges(h->st, S);
-  HIPCHK(h, hipMemsetAsync(S.counters, 0, sizeof(int) * COUNTER_INTS, h->st));
   const bool spread_stats = P.precision == 1 || P.learned;
-  if (spread_stats)
-    HIPCHK(h, hipMemsetAsync(S.spread, 0, sizeof(int) * SPREAD_SLOTS * SPREAD_STRIDE, h->st));
-  launch_uf_init(h->st, S);                       // makeSet (dcollid.cpp:1015-1031)
-  launch_scene_reset(h->st, S);                   // batched scenes: every scene's pass loop
-  if (P.record_impulses) {  // the impulses of this resolve's last pass with a hit
-    if (!S.pimp) {
-      HIPCHK(h, dalloc(&S.pimp, S.nv));
-      HIPCHK(h, dalloc(&S.pnum, S.nv));
-    }
-    HIPCHK(h, hipMemsetAsync(S.pimp, 0, sizeof(dvec4) * (size_t)S.nv, h->st));
-    HIPCHK(h, hipMemsetAsync(S.pnum, 0, sizeof(int32_t) * (size_t)S.nv, h->st));
+  if (P.record_impulses && !S.pimp) {
+    HIPCHK(h, dalloc(&S.pimp, S.nv));
+    HIPCHK(h, dalloc(&S.pnum, S.nv));
   }
-  HIPCHK(h, hipMemsetAsync(S.zmark, 0, sizeof(int32_t) * (size_t)S.nv, h->st));
-  launch_average_velocity(h->st, S, P.dt);        // computeAverageVelocity
+  if (with_predict_graph) {
+    if ((rc = ensure_step_graph(h, spread_stats))) return rc;
+    mark(h, &t0);
+    HIPCHK(h, hipGraphLaunch(h->graph_exec, h->st));
+  } else {
+    mark(h, &t0);
+    resolve_prologue(h, spread_stats);
+  }
+  HIPCHK(h, hipGetLastError());
   int pass_id = 0;
   bool any = false;
   int status = CS_OK;
@@ -1338,19 +1401,26 @@ int cs_resolve(cs_solver* h, cs_step_report* report) {
   return status;
 }
 
+}  // namespace
+
+int cs_resolve(cs_solver* h, cs_step_report* report) { return resolve_impl(h, report, false); }
+
 int cs_step(cs_solver* h, cs_step_report* report) {
+  if (h && h->P.graph_step) return resolve_impl(h, report, true);
   int rc = cs_predict(h);
   if (rc) return rc;
   return cs_resolve(h, report);
 }
 
-int cs_resolve_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
-                     int32_t nscene) {
+namespace {
+
+int resolve_batch_impl(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
+                       int32_t nscene, bool with_predict_graph) {
   if (!h) return CS_ERR_ARG;
   if (!h->assembled) return fail(h, CS_ERR_ARG, "cs_assemble has not been called");
   if (h->S.nscene <= 1 || nscene != h->S.nscene || !per_scene)
     return fail(h, CS_ERR_ARG, "nscene must equal the assembled batch's scene count");
-  const int rc = cs_resolve(h, report);
+  const int rc = resolve_impl(h, report, with_predict_graph);
   if (rc != CS_OK && rc != CS_WARN_ZONE_CAP) return rc;
   std::vector<int32_t> f((size_t)SCN_FIELDS * nscene);
   HIPCHK(h, hipMemcpyAsync(f.data(), h->S.scn, sizeof(int32_t) * f.size(), hipMemcpyDeviceToHost,
@@ -1369,8 +1439,16 @@ int cs_resolve_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_
   return rc;
 }
 
+}  // namespace
+
+int cs_resolve_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
+                     int32_t nscene) {
+  return resolve_batch_impl(h, report, per_scene, nscene, false);
+}
+
 int cs_step_batch(cs_solver* h, cs_step_report* report, cs_scene_report* per_scene,
                   int32_t nscene) {
+  if (h && h->P.graph_step) return resolve_batch_impl(h, report, per_scene, nscene, true);
   int rc = cs_predict(h);
   if (rc) return rc;
   return cs_resolve_batch(h, report, per_scene, nscene);

@@ -251,6 +251,7 @@ enum {
   SCN_FIELDS
 };
 void launch_scene_reset(hipStream_t st, DevState& S);
+void launch_fill_u32(hipStream_t st, void* p, int64_t words, uint32_t v);
 // after a pass's narrow phase: scene of each of its nhp hit pairs, then the
 // per-scene state machine of the reference's pass loop
 void launch_scene_pass(hipStream_t st, DevState& S, int kind, int index, int nhp, int max_ccd);

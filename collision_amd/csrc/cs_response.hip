@@ -643,6 +643,20 @@ void launch_final_velocity(hipStream_t st, DevState& S) {
                      S.has, S.nv);
 }
 
+// p[0..n) = v, 32-bit words (a kernel rather than hipMemsetAsync, so that a
+// captured graph holds kernel nodes only)
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+void launch_fill_u32(hipStream_t st, void* p, int64_t words, uint32_t v) {
+  if (words <= 0) return;
+  const int64_t blocks = std::min<int64_t>((words + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, (uint32_t*)p, words, v);
+}
+
 void launch_predict(hipStream_t st, DevState& S, double dt) {
   hipLaunchKernelGGL(k_predict, dim3((S.nv + 255) / 256), dim3(256), 0, st, S.x, S.x_old, S.vel,
                      dt, S.nv);

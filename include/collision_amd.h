@@ -80,7 +80,10 @@ typedef struct cs_params {
     int32_t precision_stats;    /* 1: with precision = 1, also evaluate the FP64
                                    quantities of every screened sub-test and report the
                                    largest FP32 error (cs_step_report.mx_*) */
-    int32_t pad1;
+    int32_t graph_step;         /* 1: cs_step / cs_step_batch replay one captured HIP graph
+                                   of the predictor and the resolve prologue (every launch
+                                   before the step's first host synchronisation) instead
+                                   of launching them one by one; results are identical */
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and

@@ -59,3 +59,45 @@ def test_batched_assembly_rejects_shared_vertices():
     sc.elem_scene = bad
     with pytest.raises(CollisionError):
         solver_for_scene(sc)
+
+
+@pytest.mark.parametrize("batched", [False, True], ids=["config4_96", "batch6"])
+def test_graph_step_equals_eager_step(batched):
+    """graph_step = 1: the predictor and the resolve prologue replay one
+    captured graph each step; every report and state bit equals the eager
+    launches, also across a re-capture (record_impulses switched on midway)
+    and with host reads of the state between replays."""
+    if batched:
+        sc = S.batched_scenes(count=6, nmin=16, nmax=24, steps=40)
+        sc.x = sc.x + sc.vel * (sc.dt * 150)
+    else:
+        sc = S.combined_scene(n=96, nstrings=60, steps=40)
+    ga = solver_for_scene(sc, graph_step=1, log_pairs=0)
+    gb = solver_for_scene(sc, graph_step=0, log_pairs=0)
+    keys = ("status", "n_passes", "n_ccd_passes", "n_zone_passes", "has_collision", "n_zones",
+            "strain_sweeps", "hit_pairs", "hit_subtests")
+    hits = 0
+    try:
+        for k in range(40):
+            if k == 20:
+                for g in (ga, gb):
+                    g.params.record_impulses = 1
+                    g._push()
+            if batched:
+                (ra, pa), (rb, pb) = ga.stepBatch(), gb.stepBatch()
+                assert pa == pb, k
+            else:
+                ra, rb = ga.step(), gb.step()
+            da, db = ra.as_dict(), rb.as_dict()
+            assert {x: da[x] for x in keys} == {x: db[x] for x in keys}, k
+            hits += ra.hit_pairs
+            sa, sb = ga.getState(), gb.getState()
+            for key in ("x", "vel", "avg_vel", "x_old", "has_collsn"):
+                assert np.array_equal(sa[key], sb[key]), (k, key)
+            if k >= 20:
+                ia, ib = ga.lastImpulses(), gb.lastImpulses()
+                assert np.array_equal(ia[0], ib[0]) and np.array_equal(ia[1], ib[1]), k
+    finally:
+        ga.close()
+        gb.close()
+    assert batched or hits > 0
