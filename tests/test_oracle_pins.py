@@ -23,7 +23,12 @@ wrong threshold or a wrong loop bound fails it:
   MACH_EPS             which decisions of the config-0 contact window change
                        when FronTier's MACH_EPS is not DBL_EPSILON;
   math modes           deterministic acos/cos/sin/cbrt and glibc libm give the
-                       same hit sets on configs 1 and 4 (reduced) and the decks.
+                       same hit sets on configs 1 and 4 (reduced) and the decks;
+  movable bodies       a free body translates exactly and keeps its centre of
+                       mass (updateFinalForRG touches collided bodies only);
+                       two bodies in contact keep m1 v1 + m2 v2 (the RG
+                       impulses' mass split, dcollid3d.cpp:966-972) and stay
+                       rigid.
 """
 from fractions import Fraction
 
@@ -429,3 +434,74 @@ def test_libm_and_deterministic_math_agree_on_more_scenes(O, maker, steps, min_s
         a.close()
         b.close()
     assert compared >= min_steps, compared
+
+
+# ---------------------------------------------------------------------------
+# movable rigid bodies (createImpZoneForRG dcollid3d.cpp:54-68, the RG impulse
+# branches dcollid3d.cpp:959-984, SpreadImpactZoneImpulse dcollid.cpp:1101-1115,
+# updateImpactZoneVelocityForRG and updateFinalForRG dcollid.cpp:267-288,
+# 626-675)
+def _two_boxes(v1, v2, m1, m2, c2=(0.1015, 0.013, 0.007), dt=1.0 / 1024):
+    b = S._Builder()
+    xa, ta = S.box_mesh((0.0, 0.0, 0.0), (0.1, 0.1, 0.1))
+    b.add_surface(xa, np.tile(v1, (len(xa), 1)), ta, fixed=False, movable_mass=m1)
+    xb, tb = S.box_mesh(c2, (0.1, 0.1, 0.1))
+    b.add_surface(xb, np.tile(v2, (len(xb), 1)), tb, fixed=False, movable_mass=m2)
+    return b.build("two_boxes", thickness=1e-3, dt=dt, steps=1,
+                   domain_lo=(-5.0, -5.0, -5.0), domain_hi=(5.0, 5.0, 5.0))
+
+
+def test_free_movable_body_translates_exactly(oracle_lib):
+    """A body with no contact keeps its velocity (its zone's rigid update of a
+    pure translation is exact), its points move by dt v per step (dyadic
+    values: exactly), and its centre of mass is left alone: updateFinalForRG
+    only updates bodies with a collided point."""
+    O = oracle_lib
+    v = np.array([0.25, -0.5, 0.125])
+    sc = _two_boxes(v, np.zeros(3), 1.0, 1.0, c2=(2.0, 2.0, 2.0))
+    s = O.OracleSolver(sc)
+    com0, cv0 = s.bodies()
+    for _ in range(8):
+        r = s.step()
+        assert r.status == 0 and r.hit_pairs == 0
+    com, cv = s.bodies()
+    st = s.get_state()
+    s.close()
+    assert np.array_equal(com, com0) and np.array_equal(cv, cv0)
+    n = len(sc.x) // 2
+    assert np.array_equal(st["x"][:n], sc.x[:n] + 8 * sc.dt * v)
+    assert np.array_equal(st["avg_vel"][:n], np.tile(v, (n, 1)))
+    assert not st["has_collsn"].any()
+
+
+@pytest.mark.parametrize("m1,m2", [(1.0, 3.0), (2.0, 0.5)])
+def test_movable_bodies_in_contact_conserve_momentum(oracle_lib, m1, m2):
+    """Two movable bodies meeting head on: the RG impulses split vn in the
+    ratio of the masses (m2 / (m1 + m2) to the triangle's body, m1 / (m1 + m2)
+    to the point's), so m1 v1 + m2 v2 is kept through every step while the
+    approach stops, and each body stays rigid.  A swapped mass ratio or a
+    wrong sign breaks the momentum balance; both bodies are checked through
+    their points' mean velocity (the centre-of-mass velocity of equal point
+    masses, which the rigid update keeps)."""
+    O = oracle_lib
+    v1, v2 = np.array([0.5, 0.0, 0.0]), np.array([-0.5, 0.0, 0.0])
+    sc = _two_boxes(v1, v2, m1, m2)
+    s = O.OracleSolver(sc)
+    p0 = m1 * v1 + m2 * v2
+    n = len(sc.x) // 2
+    d0a = np.linalg.norm(sc.x[:n, None] - sc.x[None, :n], axis=2)
+    hits = 0
+    for _ in range(6):
+        r = s.step()
+        assert r.status == 0
+        hits += r.hit_pairs
+        av = s.get_state()["avg_vel"]
+        va, vb = av[:n].mean(axis=0), av[n:].mean(axis=0)  # body velocities (equal point masses)
+        p = m1 * va + m2 * vb
+        assert np.allclose(p, p0, rtol=0, atol=1e-13), (p, p0)
+    st = s.get_state()
+    s.close()
+    assert hits > 0
+    assert va[0] - vb[0] < 0.5  # the approach slowed down
+    da = np.linalg.norm(st["x"][:n, None] - st["x"][None, :n], axis=2)
+    assert np.allclose(da, d0a, rtol=0, atol=1e-12)
