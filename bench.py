@@ -43,6 +43,14 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 # algorithmic FLOPs per sample of the 24-128-128-2 network
 MLP_FWD_FLOP = 2 * (24 * 128 + 128 * 128 + 128 * 2)                    # 39,424
 MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (2 * 128 * 2 + 2 * 128 * 128 + 128 * 24)  # + backward
+ROOFLINE_NOTES = {
+    "strain": "one work-group runs about 2,900 dependent level iterations per step (the exact "
+              "Gauss-Seidel order); bound by that chain's latency, not by HBM: it moves about 1.9e7 B "
+              "per launch (profiles/traffic_strain.json) against the 1.6e8 B bound of reading every "
+              "vertex and element once",
+    "traverse": "per-lane stackless walks, bound by dependent L2 misses",
+    "narrow": "gather of each sub-test's four vertices; FP64 VALU co-bound",
+}
 FORMS = {"plain": {}, "mixed": {"precision": 1}, "learned": {"learned": 1},
          "learned_mixed": {"learned": 1, "precision": 1}}
 
@@ -449,6 +457,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": dom["bytes"],
             "avg_launch_ms": dom["avg_launch_ms"],
+            "note": ROOFLINE_NOTES.get(dom_name),
         },
     }
     if not args.no_cpu:
