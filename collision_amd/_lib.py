@@ -39,7 +39,8 @@ class Params(C.Structure):
                 ("log_pairs", C.c_int32), ("device", C.c_int32),
                 ("force_full_passes", C.c_int32), ("pass_timing", C.c_int32),
                 ("record_impulses", C.c_int32), ("precision", C.c_int32),
-                ("learned", C.c_int32), ("precision_stats", C.c_int32), ("graph_step", C.c_int32)]
+                ("learned", C.c_int32), ("precision_stats", C.c_int32), ("graph_step", C.c_int32),
+                ("autotune", C.c_int32), ("pad2", C.c_int32)]
 
 
 class MeshView(C.Structure):
@@ -77,6 +78,14 @@ class StepReport(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("traverse_blocks_per_cu", C.c_int32), ("narrow_threads", C.c_int32),
+                ("query_grid", C.c_int32), ("tuned", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SceneReport(C.Structure):
@@ -161,6 +170,8 @@ def lib(path: str = LIB_PATH):
         "cs_learned_dataset": ([vp, P(C.c_float), P(C.c_float), P(C.c_float), i64], C.c_int),
         "cs_learned_train": ([vp, i32, i32, C.c_float, C.c_uint64, P(C.c_float)], C.c_int),
         "cs_learned_get_stats": ([vp, P(LearnedStats)], C.c_int),
+        "cs_get_tuning": ([vp, P(Tuning)], C.c_int),
+        "cs_set_tuning": ([vp, P(Tuning)], C.c_int),
         "cs_last_error": ([vp], C.c_char_p),
         "cs_version": ([], C.c_char_p),
     }

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -112,7 +113,53 @@ struct cs_solver {
     }
   } graph_key;
   int64_t assemble_gen = 0;  // bumped by every cs_assemble / cs_load
+  // autotuner: over its first steps every tuning step runs one candidate
+  // launch shape per kernel; per-unit kernel times are averaged per candidate
+  struct Tuner {
+    bool active = false;
+    int step = 0;
+    int tuned = 0;  // cs_tuning.tuned
+    double t_trav[4] = {}, t_narrow[3] = {}, t_query[4] = {};
+    int n_trav[4] = {}, n_narrow[3] = {}, n_query[4] = {};
+    std::string key;
+  } tuner;
 };
+
+namespace {
+constexpr int kTuneTrav[4] = {4, 8, 16, 32};
+constexpr int kTuneNarrow[3] = {64, 128, 256};
+constexpr int kTuneQuery[4] = {1024, 2048, 4096, 8192};
+constexpr int kTuneSteps = 8;  // two rounds of the four-candidate lists
+
+// CS_TUNE_CACHE: one line per key, "key traverse narrow query"
+bool tune_cache_lookup(const std::string& key, int* tr, int* na, int* qu) {
+  const char* path = std::getenv("CS_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  char k[256];
+  int a, b, c;
+  bool found = false;
+  while (std::fscanf(f, "%255s %d %d %d", k, &a, &b, &c) == 4)
+    if (key == k) {  // the last entry of a key wins
+      *tr = a;
+      *na = b;
+      *qu = c;
+      found = true;
+    }
+  std::fclose(f);
+  return found;
+}
+
+void tune_cache_store(const std::string& key, int tr, int na, int qu) {
+  const char* path = std::getenv("CS_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s %d %d %d\n", key.c_str(), tr, na, qu);
+  std::fclose(f);
+}
+}  // namespace
 
 namespace {
 
@@ -1131,6 +1178,22 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   }
   h->assembled = true;
   h->assemble_gen++;  // buffers changed: a captured step graph is stale
+  {  // launch shapes: the cache entry of this device, size and batch, or the defaults
+    hipDeviceProp_t prop;
+    std::string arch = "gpu";
+    if (hipGetDeviceProperties(&prop, h->P.device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& ch : arch)
+      if (ch == ' ' || ch == ':') ch = '_';
+    int lg = 0;
+    while (lg < 40 && (1ll << (lg + 1)) <= (long long)std::max(ne, 1)) ++lg;
+    h->tuner = cs_solver::Tuner();
+    h->tuner.key = arch + ":ne2^" + std::to_string(lg) + (S.nscene > 1 ? ":batch" : ":single");
+    int tr = 8, na = 128, qu = 4096;
+    if (tune_cache_lookup(h->tuner.key, &tr, &na, &qu)) h->tuner.tuned = 2;
+    S.tune_trav = tr;
+    S.tune_narrow = na;
+    S.tune_query = qu;
+  }
   h->state_set = false;
   return CS_OK;
 }
@@ -1263,6 +1326,18 @@ int resolve_impl(cs_solver* h, cs_step_report* report, bool with_predict_graph) 
   hipEvent_t t0, t_strain0, t_end;
   std::vector<hipEvent_t> strain_marks;
   const bool spread_stats = P.precision == 1 || P.learned;
+  cs_solver::Tuner& tu = h->tuner;
+  if (P.autotune && !tu.active && tu.tuned == 0 && tu.step == 0) tu.active = true;
+  int saved_timing = -1;
+  if (tu.active) {  // this step's candidates; kernel times need the timing markers
+    S.tune_trav = kTuneTrav[tu.step % 4];
+    S.tune_narrow = kTuneNarrow[tu.step % 3];
+    S.tune_query = kTuneQuery[tu.step % 4];
+    if (h->P.pass_timing < 1) {
+      saved_timing = h->P.pass_timing;
+      h->P.pass_timing = 1;
+    }
+  }
   if (P.record_impulses && !S.pimp) {
     HIPCHK(h, dalloc(&S.pimp, S.nv));
     HIPCHK(h, dalloc(&S.pnum, S.nv));
@@ -1333,6 +1408,41 @@ int resolve_impl(cs_solver* h, cs_step_report* report, bool with_predict_graph) 
     pi.ms_response = elapsed(T.e2, T.e3);
     pi.ms_traverse_kernel = elapsed(T.t0, T.t1);
     pi.ms_narrow_kernel = elapsed(T.n0, T.n1);
+  }
+  if (tu.active) {  // per-unit kernel times of this step's candidates
+    double trav = 0.0, query = 0.0, narrow = 0.0;
+    int64_t lanes = 0;
+    int nfull = 0, ninc = 0;
+    for (const cs_pass_info& pi : h->passes) {
+      if (pi.incremental) {
+        query += pi.ms_traverse_kernel;
+        ++ninc;
+      } else {
+        trav += pi.ms_traverse_kernel;
+        ++nfull;
+      }
+      narrow += pi.ms_narrow_kernel;
+      lanes += std::max<int64_t>(pi.tested_pairs, 0);
+    }
+    const int a = tu.step % 4, b = tu.step % 3;
+    if (nfull) { tu.t_trav[a] += trav / nfull; tu.n_trav[a]++; }
+    if (ninc) { tu.t_query[a] += query / ninc; tu.n_query[a]++; }
+    if (lanes) { tu.t_narrow[b] += narrow / (double)lanes; tu.n_narrow[b]++; }
+    if (saved_timing >= 0) h->P.pass_timing = saved_timing;
+    if (++tu.step >= kTuneSteps) {  // keep the fastest of each (defaults where unmeasured)
+      auto best = [](const double* t, const int* n, int m, const int* cand, int def) {
+        int bi = -1;
+        for (int i = 0; i < m; ++i)
+          if (n[i] && (bi < 0 || t[i] / n[i] < t[bi] / n[bi])) bi = i;
+        return bi < 0 ? def : cand[bi];
+      };
+      S.tune_trav = best(tu.t_trav, tu.n_trav, 4, kTuneTrav, 8);
+      S.tune_narrow = best(tu.t_narrow, tu.n_narrow, 3, kTuneNarrow, 128);
+      S.tune_query = best(tu.t_query, tu.n_query, 4, kTuneQuery, 4096);
+      tu.active = false;
+      tu.tuned = 1;
+      tune_cache_store(tu.key, S.tune_trav, S.tune_narrow, S.tune_query);
+    }
   }
   if (status == CS_OK || status == CS_WARN_ZONE_CAP) {
     R.strain_sweeps = h->h_counters[CNT_STRAIN_SWEEPS];
@@ -1815,6 +1925,33 @@ int cs_learned_get_stats(cs_solver* h, cs_learned_stats* out) {
   if (!h || !out) return CS_ERR_ARG;
   *out = h->lstats;
   h->lstats = cs_learned_stats{};
+  return CS_OK;
+}
+
+int cs_get_tuning(cs_solver* h, cs_tuning* out) {
+  if (!h || !out) return CS_ERR_ARG;
+  out->traverse_blocks_per_cu = h->S.tune_trav;
+  out->narrow_threads = h->S.tune_narrow;
+  out->query_grid = h->S.tune_query;
+  out->tuned = h->tuner.tuned;
+  return CS_OK;
+}
+
+int cs_set_tuning(cs_solver* h, const cs_tuning* in) {
+  if (!h || !in) return CS_ERR_ARG;
+  auto one_of = [](int v, const int* c, int n) {
+    for (int i = 0; i < n; ++i)
+      if (c[i] == v) return true;
+    return false;
+  };
+  if (!one_of(in->traverse_blocks_per_cu, kTuneTrav, 4) || !one_of(in->narrow_threads, kTuneNarrow, 3) ||
+      !one_of(in->query_grid, kTuneQuery, 4))
+    return fail(h, CS_ERR_ARG, "launch shape outside the candidate lists");
+  h->S.tune_trav = in->traverse_blocks_per_cu;
+  h->S.tune_narrow = in->narrow_threads;
+  h->S.tune_query = in->query_grid;
+  h->tuner.active = false;
+  h->tuner.tuned = 0;
   return CS_OK;
 }
 

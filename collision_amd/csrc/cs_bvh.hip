@@ -401,7 +401,6 @@ constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flu
 // vertex always overlaps (both static boxes hold that vertex), a pair on one
 // rigid surface is tested.  The kept pairs are tested by k_split_seed, so
 // one walk serves both passes.
-constexpr int PT_BLOCKS_PER_CU = 8;
 constexpr int PT_REFILL = 16;   // refill once this many lanes are idle
 constexpr int PT_STEPS = 16;    // walk steps between refill checks
 
@@ -1103,7 +1102,7 @@ void launch_grow_dirty(hipStream_t st, DevState& S, double dt) {
 void launch_query_dirty(hipStream_t st, DevState& S, int emit) {
   if (S.ne < 2) return;
   const int npre = 2 * S.ne - 1;
-  hipLaunchKernelGGL(k_query_ranges, dim3(std::min((npre + 63) / 64, 4096)), dim3(64), 0, st,
+  hipLaunchKernelGGL(k_query_ranges, dim3(std::min((npre + 63) / 64, S.tune_query)), dim3(64), 0, st,
                      S.nodes, S.nright, S.ebox, S.ev, S.eflags, S.dirty_e, S.dpre, S.rlist, npre,
                      emit, S.nscene > 1, S.qpairs, S.qpair_cap, S.counters);
 }
@@ -1201,7 +1200,7 @@ void launch_traverse(hipStream_t st, DevState& S, int2* out, int cap, const dbox
   const int n = S.ne;
   if (n < 2) return;
   (void)hipMemsetAsync(S.tnext, 0, sizeof(int) * kNumXcd, st);
-  const int blocks = std::min(S.num_cus * PT_BLOCKS_PER_CU, (n + TRAV_THREADS - 1) / TRAV_THREADS);
+  const int blocks = std::min(S.num_cus * S.tune_trav, (n + TRAV_THREADS - 1) / TRAV_THREADS);
   if (pbox)
     hipLaunchKernelGGL(k_traverse<true>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
                        S.nodes, S.lpre, S.ebox, pbox, S.ev, S.eflags, n, out, cap,

@@ -184,6 +184,9 @@ struct DevState {
   int32_t *rflag = nullptr, *rlist = nullptr;
   int32_t* tnext = nullptr;        // persistent full traversal: next leaf per XCD share
   int num_cus = 256;               // compute units of the device
+  // launch shapes (cs_set_tuning / the autotuner): persistent traversal
+  // work-groups per CU, narrow-phase work-group size, dirty-query grid cap
+  int tune_trav = 8, tune_narrow = 128, tune_query = 4096;
   // candidate stream (current and previous pass)
   int2* pairs = nullptr;
   int2* pairs_prev = nullptr;

@@ -25,7 +25,6 @@
 
 namespace cs {
 
-constexpr int NARROW_THREADS = 128;
 constexpr int SUB_GROUP = 16;  // lanes per pair (sub-tests 0..14)
 // learned form: codes of the per-slot seeds (a scored slot holds its seed in
 // [0, 1], sign bit set when a root is predicted) and the 8-bit pair keys
@@ -1042,7 +1041,8 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
 
 // npairs_dev != nullptr: the pair count is read on the device (clamped to
 // cap); the grid covers the first npairs pairs of it
-__global__ void __launch_bounds__(NARROW_THREADS, 3)
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS, 3)
     k_narrow(const int2* __restrict__ pairs, int npairs, const int* __restrict__ npairs_dev,
              int cap, const int4* __restrict__ ev, const dvec4* __restrict__ x_old,
              const dvec4* __restrict__ av, const uint8_t* __restrict__ fixed, NarrowParams P,
@@ -1053,10 +1053,10 @@ __global__ void __launch_bounds__(NARROW_THREADS, 3)
   // 8 XCDs round-robin, so the first `busy` of them (those the pair count
   // needs; with a count read on the device the grid may be larger) are spread
   // evenly, and remapping within them gives each XCD one contiguous share
-  const int busy = (int)(((int64_t)n * SUB_GROUP + NARROW_THREADS - 1) / NARROW_THREADS);
+  const int busy = (int)(((int64_t)n * SUB_GROUP + THREADS - 1) / THREADS);
   if ((int)blockIdx.x >= busy) return;
   const int b = xcd_block(blockIdx.x, busy);
-  const int64_t gl = (int64_t)b * NARROW_THREADS + threadIdx.x;
+  const int64_t gl = (int64_t)b * THREADS + threadIdx.x;
   if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
   narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
 }
@@ -1071,10 +1071,16 @@ void launch_narrow(hipStream_t st, DevState& S, const NarrowParams& P, const int
                    int npairs, const int* npairs_dev, int cap) {
   if (npairs <= 0) return;
   const int64_t lanes = (int64_t)npairs * SUB_GROUP;
-  const int64_t blocks = (lanes + NARROW_THREADS - 1) / NARROW_THREADS;
-  hipLaunchKernelGGL(k_narrow, dim3((unsigned)blocks), dim3(NARROW_THREADS), 0, st, pairs, npairs,
-                     npairs_dev, cap, S.ev, S.x_old, S.av, S.fixed, P, zone_list(S), S.hits,
-                     S.hit_cap, S.hp, S.hp_cap, S.counters);
+  const int T = S.tune_narrow;
+  const int64_t blocks = (lanes + T - 1) / T;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(T), 0, st, pairs, npairs, npairs_dev,
+                       cap, S.ev, S.x_old, S.av, S.fixed, P, zone_list(S), S.hits, S.hit_cap, S.hp,
+                       S.hp_cap, S.counters);
+  };
+  if (T == 64) go(k_narrow<64>);
+  else if (T == 256) go(k_narrow<256>);
+  else go(k_narrow<128>);
 }
 
 // Learned form, step 1: the certified coplanarity screen of every CCD

@@ -177,6 +177,16 @@ class CollisionSolver3d:
         self.predict()
         return self.resolveCollision()
 
+    # -- launch shapes (no counterpart in the reference) --
+    def tuning(self) -> dict:
+        t = _lib.Tuning()
+        self._check(self.L.cs_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def setTuning(self, traverse_blocks_per_cu, narrow_threads, query_grid):
+        t = _lib.Tuning(traverse_blocks_per_cu, narrow_threads, query_grid, 0)
+        self._check(self.L.cs_set_tuning(self.h, C.byref(t)))
+
     # -- batched scenes (no counterpart in the reference) --
     def resolveBatch(self):
         """cs_resolve_batch: (batch report, list of per-scene report dicts)"""

@@ -84,6 +84,13 @@ typedef struct cs_params {
                                    of the predictor and the resolve prologue (every launch
                                    before the step's first host synchronisation) instead
                                    of launching them one by one; results are identical */
+    int32_t autotune;           /* 1: the first 8 steps after cs_assemble time launch-shape
+                                   candidates (cs_tuning) and keep the fastest; with the
+                                   environment variable CS_TUNE_CACHE naming a file, the
+                                   choice is stored there per device, size and batch, and
+                                   a later assembly of the same key takes it without tuning.
+                                   Results do not depend on the launch shapes. */
+    int32_t pad2;
 } cs_params;
 
 /* Flat mesh: replaces assembleFromInterface's walk over FronTier surfaces and
@@ -307,6 +314,19 @@ int cs_learned_train(cs_solver *h, int32_t epochs, int32_t batch, float lr, uint
                      float *loss);
 /* statistics of the learned form since the previous call (which resets them) */
 int cs_learned_get_stats(cs_solver *h, cs_learned_stats *out);
+
+/* ---- launch shapes (no counterpart in the reference) ----
+ * persistent traversal work-groups per CU (4, 8, 16, 32), narrow-phase
+ * work-group size (64, 128, 256) and the dirty-element query's grid cap
+ * (1024 .. 8192); the results of a step do not depend on them */
+typedef struct cs_tuning {
+    int32_t traverse_blocks_per_cu;
+    int32_t narrow_threads;
+    int32_t query_grid;
+    int32_t tuned;             /* 0 defaults or set, 1 chosen by the autotuner, 2 from the cache */
+} cs_tuning;
+int cs_get_tuning(cs_solver *h, cs_tuning *out);
+int cs_set_tuning(cs_solver *h, const cs_tuning *in);
 
 /* ---- diagnostics and restart ---- */
 /* the summed collsnImpulse [nv][3] and collsn_num [nv] every vertex received

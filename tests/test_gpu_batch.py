@@ -101,3 +101,38 @@ def test_graph_step_equals_eager_step(batched):
         ga.close()
         gb.close()
     assert batched or hits > 0
+
+
+def test_autotuner_keeps_results_and_caches_its_choice(tmp_path, monkeypatch):
+    """autotune = 1: the first 8 steps run candidate launch shapes; the state
+    equals an untuned run bit for bit every step, the choice is kept, stored
+    in CS_TUNE_CACHE and taken by the next assembly of the same key."""
+    from collision_amd.solver import CollisionError
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("CS_TUNE_CACHE", str(cache))
+    sc = S.combined_scene(n=96, nstrings=60, steps=30)
+    ga = solver_for_scene(sc, autotune=1, log_pairs=0)
+    gb = solver_for_scene(sc, log_pairs=0)
+    assert ga.tuning()["tuned"] == 0
+    try:
+        for k in range(30):
+            ra, rb = ga.step(), gb.step()
+            assert ra.hit_pairs == rb.hit_pairs and ra.n_passes == rb.n_passes, k
+            sa, sb = ga.getState(), gb.getState()
+            for key in ("x", "vel", "avg_vel", "has_collsn"):
+                assert np.array_equal(sa[key], sb[key]), (k, key)
+        t = ga.tuning()
+        print(f"[tune] {t}", flush=True)
+        assert t["tuned"] == 1
+        assert cache.exists()
+        gc = solver_for_scene(sc, log_pairs=0)
+        tc = gc.tuning()
+        assert tc["tuned"] == 2 and {k: tc[k] for k in ("traverse_blocks_per_cu", "narrow_threads",
+                                                        "query_grid")} == \
+            {k: t[k] for k in ("traverse_blocks_per_cu", "narrow_threads", "query_grid")}
+        with pytest.raises(CollisionError):
+            gc.setTuning(8, 96, 4096)
+        gc.close()
+    finally:
+        ga.close()
+        gb.close()
