@@ -206,6 +206,11 @@ void free_dev(DevState& S) {
   dfree(S.dirty_e); dfree(S.dlist); dfree(S.pbox); dfree(S.pairs_seed);
   dfree(S.vlist); dfree(S.vadj_ptr); dfree(S.vadj);
   dfree(S.dpre); dfree(S.rflag); dfree(S.rlist); dfree(S.tnext);
+  {
+    uint8_t* tq = static_cast<uint8_t*>(S.tq);
+    dfree(tq);
+    S.tq = nullptr;
+  }
   dfree(S.hkey); dfree(S.hkey_tmp); dfree(S.hidx); dfree(S.hidx_tmp);
   dfree(S.rkey); dfree(S.rkey_tmp); dfree(S.rval); dfree(S.rval_tmp);
   dfree(S.zmark); dfree(S.zlist); dfree(S.zkey64); dfree(S.zkey64_tmp);
@@ -1071,7 +1076,12 @@ int cs_assemble(cs_solver* h, const cs_mesh_view* m) {
   HIPCHK(h, dalloc(&S.vadj_ptr, (size_t)nv + 1));
   HIPCHK(h, dalloc(&S.vadj, vadj.size()));
   HIPCHK(h, dalloc(&S.dpre, 2 * ne - 1));
-  HIPCHK(h, dalloc(&S.tnext, kNumXcd));
+  HIPCHK(h, dalloc(&S.tnext, kNumXcd + 2));
+  {
+    uint8_t* tq = nullptr;
+    HIPCHK(h, dalloc(&tq, 2 * TRAV_QCAP * TRAV_ITEM_BYTES));
+    S.tq = tq;
+  }
   {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->P.device) == hipSuccess &&

@@ -403,6 +403,26 @@ constexpr int TRAV_BUF = 8;  // pairs buffered per lane before an individual flu
 // one walk serves both passes.
 constexpr int PT_REFILL = 16;   // refill once this many lanes are idle
 constexpr int PT_STEPS = 16;    // walk steps between refill checks
+//
+// Long walks.  A few leaves have boxes that overlap a large part of the
+// mesh (a static obstacle's big faces under a cloth): their walks visit tens
+// of thousands of nodes one dependent load after another, and the kernel
+// waits for the longest.  A walk that reaches TRAV_LONG steps hands the rest
+// of its pre-order range [nd, end) to k_traverse_split, where TRAV_SPLIT
+// lanes each walk one sub-range.  A rope walk started at any node x visits
+// x's subtree and then the following subtrees, i.e. exactly the nodes from x
+// on, so walks over [s0, s1) and [s1, end) find the pairs of the walk over
+// [s0, end), each once.
+constexpr int TRAV_LONG = 512;
+constexpr int TRAV_SPLIT = 64;
+
+// The rest of one walk: element e over the pre-order nodes [lo, hi), with the
+// FP32 box of its leaf.
+struct __align__(16) TravItem {
+  int e, lo, hi, pad;
+  float blo[4], bhi[4];
+};
+static_assert(sizeof(TravItem) == TRAV_ITEM_BYTES, "TravItem layout");
 
 template <bool DUAL>
 __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __restrict__ nodes,
@@ -415,8 +435,11 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
                                                            int* __restrict__ pair_count,
                                                            unsigned long long* __restrict__ cand_count,
                                                            unsigned long long* __restrict__ pcand_count,
-                                                           int* __restrict__ next, int scenes) {
+                                                           int* __restrict__ next, int scenes,
+                                                           TravItem* __restrict__ q, int qcap) {
   __shared__ int2 buf[TRAV_BUF * TRAV_THREADS];
+  const int nnodes = 2 * n - 1;
+  int* qn = next + kNumXcd;  // items handed to the split kernel
   const int t = threadIdx.x, lane = t & 63;
   const uint64_t below = (1ull << lane) - 1ull;
   // the eighth of the leaves this block's XCD starts on (blocks are dealt
@@ -427,6 +450,7 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
   int nbuf = 0;
   int nd = -1;  // current node of this lane's walk, -1: idle
   int e = 0;
+  int wlen = 0;  // steps of the current walk
   float mlo[3] = {0.f, 0.f, 0.f}, mhi[3] = {0.f, 0.f, 0.f};
   dbox mb{};
   int4 mt{};
@@ -484,12 +508,23 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
             mhi[d] = me.hi[d];
           }
           nd = me.rope;
+          wlen = 0;
         }
       }
     }
     if (!more && __ballot(nd >= 0) == 0) break;
     // walk
     for (int step = 0; step < PT_STEPS && nd >= 0; ++step) {
+      if (wlen == TRAV_LONG) {  // hand the rest of this walk over
+        const int slot = atomicAdd(qn, 1);
+        if (slot < qcap) {
+          q[slot] = TravItem{e, nd, nnodes, 0, {mlo[0], mlo[1], mlo[2], 0.f}, {mhi[0], mhi[1], mhi[2], 0.f}};
+          nd = -1;
+          break;
+        }
+        // (no room: this lane walks on)
+      }
+      ++wlen;
       const bvh_node node = nodes[nd];
       if (!f32_overlap(node, mlo, mhi)) {
         nd = node.rope;
@@ -524,6 +559,115 @@ __global__ void __launch_bounds__(TRAV_THREADS) k_traverse(const bvh_node* __res
     if (__ballot(nbuf == TRAV_BUF)) flush();
   }
   flush();
+  unsigned long long c = cand;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0 && c) atomicAdd(cand_count, c);
+  if constexpr (DUAL) {
+    unsigned long long pc = pcand;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pc += __shfl_xor(pc, o);
+    if (lane == 0 && pc) atomicAdd(pcand_count, pc);
+  }
+}
+
+// Split walks: lane g walks sub-range g % TRAV_SPLIT of item g / TRAV_SPLIT.
+// A sub-range walk that again reaches TRAV_LONG steps is handed to the next
+// round (qout), if there is one.  Pairs are written with one atomic per
+// wavefront and step.
+template <bool DUAL>
+__global__ void __launch_bounds__(256) k_traverse_split(const bvh_node* __restrict__ nodes,
+                                                        const dbox* __restrict__ ebox,
+                                                        const dbox* __restrict__ pbox,
+                                                        const int4* __restrict__ ev,
+                                                        const uint8_t* __restrict__ eflags,
+                                                        const TravItem* __restrict__ qin,
+                                                        const int* __restrict__ qin_n,
+                                                        TravItem* __restrict__ qout,
+                                                        int* __restrict__ qout_n, int qcap,
+                                                        int2* __restrict__ pairs, int cap,
+                                                        int* __restrict__ pair_count,
+                                                        unsigned long long* __restrict__ cand_count,
+                                                        unsigned long long* __restrict__ pcand_count,
+                                                        int scenes) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int total = min(*qin_n, qcap) * TRAV_SPLIT;
+  unsigned long long cand = 0, pcand = 0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int g0 = blockIdx.x * blockDim.x; g0 < total; g0 += stride) {  // (uniform per block)
+    const int g = g0 + (int)threadIdx.x;
+    int nd = -1, s1 = 0, e = 0, wlen = 0;
+    float mlo[3] = {0.f, 0.f, 0.f}, mhi[3] = {0.f, 0.f, 0.f};
+    dbox mb{};
+    int4 mt{};
+    uint8_t mf = 0;
+    if (g < total) {
+      const TravItem it = qin[g / TRAV_SPLIT];
+      const int j = g % TRAV_SPLIT;
+      const int64_t span = (int64_t)it.hi - it.lo;
+      nd = it.lo + (int)(span * j / TRAV_SPLIT);
+      s1 = it.lo + (int)(span * (j + 1) / TRAV_SPLIT);
+      e = it.e;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        mlo[d] = it.blo[d];
+        mhi[d] = it.bhi[d];
+      }
+      mb = ebox[e];
+      mt = ev[e];
+      mf = eflags[e];
+    }
+    for (;;) {
+      const bool active = nd >= 0 && nd < s1;
+      if (__ballot(active) == 0) break;
+      bool emit = false;
+      int2 pr{};
+      if (active) {
+        if (wlen == TRAV_LONG && qout) {  // hand the rest of this sub-range over
+          const int slot = atomicAdd(qout_n, 1);
+          if (slot < qcap) {
+            qout[slot] = TravItem{e, nd, s1, 0, {mlo[0], mlo[1], mlo[2], 0.f}, {mhi[0], mhi[1], mhi[2], 0.f}};
+            nd = -1;
+          }
+        }
+      }
+      if (active && nd >= 0) {
+        ++wlen;
+        const bvh_node node = nodes[nd];
+        if (!f32_overlap(node, mlo, mhi)) {
+          nd = node.rope;
+        } else if (node.left < 0) {
+          ++nd;
+        } else {
+          nd = node.rope;
+          const int e2 = node.left;
+          const dbox ob = ebox[e2];
+          const int4 ot = ev[e2];
+          const uint8_t of = eflags[e2];
+          asm volatile("" ::"v"(ot.x), "v"(ot.y), "v"(ot.z), "v"(ot.w), "v"((int)of));
+          if (f64_overlap(ob, mb) && !(scenes && scene_of(ot) != scene_of(mt))) {
+            ++cand;
+            const int rk = (e > e2) ? pair_reject_kind(mt, ot, mf) : pair_reject_kind(ot, mt, of);
+            if (rk) {
+              if constexpr (DUAL)
+                if (rk == REJ_SHARED || f64_overlap(pbox[e], pbox[e2])) ++pcand;
+            } else {
+              emit = true;
+              pr = make_int2(min(e, e2), max(e, e2));
+            }
+          }
+        }
+      }
+      const uint64_t m = __ballot(emit);
+      if (m) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(pair_count, (int)__popcll(m));
+        base = __shfl(base, 0) + (int)__popcll(m & below);
+        if (emit && base < cap) pairs[base] = pr;
+      }
+    }
+  }
   unsigned long long c = cand;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
@@ -1199,18 +1343,37 @@ hipError_t refit_bvh(hipStream_t st, DevState& S) {
 void launch_traverse(hipStream_t st, DevState& S, int2* out, int cap, const dbox* pbox) {
   const int n = S.ne;
   if (n < 2) return;
-  (void)hipMemsetAsync(S.tnext, 0, sizeof(int) * kNumXcd, st);
+  // per-XCD leaf counters, then the item counts of the two split rounds
+  (void)hipMemsetAsync(S.tnext, 0, sizeof(int) * (kNumXcd + 2), st);
   const int blocks = std::min(S.num_cus * S.tune_trav, (n + TRAV_THREADS - 1) / TRAV_THREADS);
-  if (pbox)
+  TravItem* q0 = reinterpret_cast<TravItem*>(S.tq);
+  TravItem* q1 = q0 + TRAV_QCAP;
+  int* n0 = S.tnext + kNumXcd;
+  int* n1 = n0 + 1;
+  auto* cand = (unsigned long long*)(S.counters + CNT_CAND64);
+  auto* pcand = pbox ? (unsigned long long*)(S.counters + CNT_PCAND64) : nullptr;
+  const int sblocks = 2 * S.num_cus;
+  if (pbox) {
     hipLaunchKernelGGL(k_traverse<true>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
                        S.nodes, S.lpre, S.ebox, pbox, S.ev, S.eflags, n, out, cap,
-                       S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
-                       (unsigned long long*)(S.counters + CNT_PCAND64), S.tnext, S.nscene > 1);
-  else
+                       S.counters + CNT_PAIRS, cand, pcand, S.tnext, S.nscene > 1, q0, TRAV_QCAP);
+    hipLaunchKernelGGL(k_traverse_split<true>, dim3(sblocks), dim3(256), 0, st, S.nodes, S.ebox,
+                       pbox, S.ev, S.eflags, q0, n0, q1, n1, TRAV_QCAP, out, cap,
+                       S.counters + CNT_PAIRS, cand, pcand, S.nscene > 1);
+    hipLaunchKernelGGL(k_traverse_split<true>, dim3(sblocks), dim3(256), 0, st, S.nodes, S.ebox,
+                       pbox, S.ev, S.eflags, q1, n1, nullptr, nullptr, TRAV_QCAP, out, cap,
+                       S.counters + CNT_PAIRS, cand, pcand, S.nscene > 1);
+  } else {
     hipLaunchKernelGGL(k_traverse<false>, dim3(std::max(blocks, 1)), dim3(TRAV_THREADS), 0, st,
                        S.nodes, S.lpre, S.ebox, nullptr, S.ev, S.eflags, n, out, cap,
-                       S.counters + CNT_PAIRS, (unsigned long long*)(S.counters + CNT_CAND64),
-                       nullptr, S.tnext, S.nscene > 1);
+                       S.counters + CNT_PAIRS, cand, nullptr, S.tnext, S.nscene > 1, q0, TRAV_QCAP);
+    hipLaunchKernelGGL(k_traverse_split<false>, dim3(sblocks), dim3(256), 0, st, S.nodes, S.ebox,
+                       nullptr, S.ev, S.eflags, q0, n0, q1, n1, TRAV_QCAP, out, cap,
+                       S.counters + CNT_PAIRS, cand, nullptr, S.nscene > 1);
+    hipLaunchKernelGGL(k_traverse_split<false>, dim3(sblocks), dim3(256), 0, st, S.nodes, S.ebox,
+                       nullptr, S.ev, S.eflags, q1, n1, nullptr, nullptr, TRAV_QCAP, out, cap,
+                       S.counters + CNT_PAIRS, cand, nullptr, S.nscene > 1);
+  }
 }
 
 void launch_split_seed(hipStream_t st, DevState& S, int n) {

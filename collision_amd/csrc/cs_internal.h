@@ -125,6 +125,8 @@ struct alignas(16) StrainSlot {
 };
 static_assert(sizeof(StrainSlot) == 256, "StrainSlot layout");
 constexpr int STRAIN_LEVEL_CAP = 1024;  // most slots in one level
+constexpr int TRAV_QCAP = 65536;        // split walks per round of the full traversal
+constexpr size_t TRAV_ITEM_BYTES = 48;
 
 // All device memory of one solver handle.
 struct DevState {
@@ -182,7 +184,9 @@ struct DevState {
   // dirty leaves by pre-order index, and the QW_RANGE-node ranges holding them
   uint8_t* dpre = nullptr;
   int32_t *rflag = nullptr, *rlist = nullptr;
-  int32_t* tnext = nullptr;        // persistent full traversal: next leaf per XCD share
+  int32_t* tnext = nullptr;        // persistent full traversal: next leaf per XCD share,
+                                   // then the item counts of the two split rounds
+  void* tq = nullptr;              // split walks of the full traversal, 2 x TRAV_QCAP items
   int num_cus = 256;               // compute units of the device
   // launch shapes (cs_set_tuning / the autotuner): persistent traversal
   // work-groups per CU, narrow-phase work-group size, dirty-query grid cap
