@@ -48,7 +48,8 @@ ROOFLINE_NOTES = {
               "Gauss-Seidel order); bound by that chain's latency, not by HBM: it moves about 1.9e7 B "
               "per launch (profiles/traffic_strain.json) against the 1.6e8 B bound of reading every "
               "vertex and element once",
-    "traverse": "per-lane stackless walks, bound by dependent L2 misses",
+    "traverse": "per-lane stackless walks, bound by dependent L2 misses; walks longer than 512 steps "
+                "are continued over 64 sub-ranges by k_traverse_split (its time is included)",
     "narrow": "gather of each sub-test's four vertices; FP64 VALU co-bound",
 }
 FORMS = {"plain": {}, "mixed": {"precision": 1}, "learned": {"learned": 1},
