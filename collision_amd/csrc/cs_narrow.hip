@@ -929,12 +929,18 @@ __device__ void learned_stats(const NarrowParams& P, bool active, bool coplanar,
 // rule).  Hit records take one atomic per wavefront; their order does not
 // matter (they are sorted by key before use).
 
+// One vertex of a pair as its sub-test lanes read it, staged in LDS.
+struct StagedVertex {
+  dvec4 xo, av;
+  int fixed, body, pad0, pad1;
+};
+
 __device__ __attribute__((always_inline)) inline void narrow_lane(
     int64_t gl, const int2* __restrict__ pairs, int npairs, const int4* __restrict__ ev,
     const dvec4* __restrict__ x_old, const dvec4* __restrict__ av,
     const uint8_t* __restrict__ fixed, const NarrowParams& P, const ZoneList& Z,
     hit_rec* __restrict__ hits, int hit_cap, int4* __restrict__ hp, int hp_cap,
-    int* __restrict__ counters) {
+    int* __restrict__ counters, StagedVertex* __restrict__ stage) {
   int* err = counters + CNT_ERR;
   const int64_t i = gl / SUB_GROUP;
   const int s = (int)(gl % SUB_GROUP);
@@ -959,21 +965,38 @@ __device__ __attribute__((always_inline)) inline void narrow_lane(
   double roots[4] = {-1.0, -1.0, -1.0, -1.0};
   Sub S;
   Contrib C;
+  // The pair's (up to) six vertices are loaded once per 16-lane group, lane s
+  // loading vertex g[s], and every sub-test lane reads its four from LDS.
+  // (The group is 16 lanes of one wavefront, whose LDS operations complete in
+  // order: no barrier.)
+  if (i < npairs && s < 6) {
+    int vid = g[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+      if (s == k) vid = g[k];
+    stage[s].xo = x_old[vid];
+    stage[s].av = av[vid];
+    stage[s].fixed = fixed[vid];
+    stage[s].body = P.vbody ? P.vbody[vid] : -1;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (active) {
     bool is_pt;
     subtest_points(kind, s, P.ccd, q, &is_pt);
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
+      const StagedVertex& v = stage[q[p]];
+      S.xo[p][0] = v.xo.x; S.xo[p][1] = v.xo.y; S.xo[p][2] = v.xo.z;
+      S.v[p][0] = v.av.x; S.v[p][1] = v.av.y; S.v[p][2] = v.av.z;
+      S.st[p] = v.fixed != 0;
+      S.bd[p] = v.body;
+      S.mv[p] = S.bd[p] >= 0;
       int vid = g[0];
 #pragma unroll
       for (int k = 1; k < 6; ++k)
         if (q[p] == k) vid = g[k];
-      const dvec4 xo = x_old[vid], vv = av[vid];
-      S.xo[p][0] = xo.x; S.xo[p][1] = xo.y; S.xo[p][2] = xo.z;
-      S.v[p][0] = vv.x; S.v[p][1] = vv.y; S.v[p][2] = vv.z;
-      S.st[p] = fixed[vid] != 0;
-      S.bd[p] = P.vbody ? P.vbody[vid] : -1;
-      S.mv[p] = S.bd[p] >= 0;
       q[p] = vid;  // from here on: global vertex ids
     }
     if (P.mixed && !P.ccd) {
@@ -1058,7 +1081,9 @@ __global__ void __launch_bounds__(THREADS, 3)
   const int b = xcd_block(blockIdx.x, busy);
   const int64_t gl = (int64_t)b * THREADS + threadIdx.x;
   if ((gl & ~63ll) >= (int64_t)n * SUB_GROUP) return;  // whole wavefronts past the end
-  narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters);
+  __shared__ StagedVertex stage[THREADS / SUB_GROUP][6];
+  narrow_lane(gl, pairs, n, ev, x_old, av, fixed, P, Z, hits, hit_cap, hp, hp_cap, counters,
+              stage[threadIdx.x / SUB_GROUP]);
 }
 
 static ZoneList zone_list(DevState& S) {
