@@ -52,6 +52,7 @@ enum {
   CNT_SNAP = 28,         // 28..31: pair/hit counts carried into the current pass
   CNT_PCAND64 = 32,      // 32, 33: static-box overlaps found by the swept-box traversal
   CNT_PPAIRS = 34,       // pairs of that traversal whose static boxes overlap
+  CNT_STRAIN_PROG = 35,  // strain sweep: level iteration reached (-1: done), read by its prefetcher
   CNT_INTS = 40
 };
 

@@ -780,6 +780,7 @@ __global__ void k_strain_pack(StrainSlot* __restrict__ slots, const dvec4* __res
 __global__ void k_strain_check(const StrainSlot* __restrict__ slots, double dt, int n,
                                uint32_t* __restrict__ spre, uint8_t* __restrict__ lvl_active,
                                int* __restrict__ counters) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[CNT_STRAIN_PROG] = 0;
   if (counters[CNT_STRAIN_FLAG] < 0) return;  // loop already finished
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   bool any = false;
@@ -846,6 +847,48 @@ __device__ inline int wave_sum(int x) {
   return x;
 }
 
+// The pipe's records come from HBM: each level's slots were last touched by
+// the pack and check kernels, which stream all of them.  A second work-group
+// on the pipe's XCD (work-groups are dealt to the XCDs round-robin, so block
+// kNumXcd shares block 0's) reads the records and pre-check words of the
+// next SP_AHEAD levels into that XCD's L2 while the pipe works, following the
+// iteration the pipe publishes.  It only reads, the pipe never waits for it,
+// and it stops when the pipe is done, when it has read the last level, or
+// after SP_IDLE polls without progress.
+constexpr int SP_AHEAD = 8;
+constexpr int SP_IDLE = 200000;
+
+__device__ void strain_prefetch(const int32_t* __restrict__ lvl_ptr, int n_levels,
+                                const StrainSlot* __restrict__ slots, const uint32_t* __restrict__ spre,
+                                int* __restrict__ counters) {
+  if (__hip_atomic_load(counters + CNT_STRAIN_FLAG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= 0)
+    return;  // nothing to sweep (or the sweep already ended)
+  const int tid = threadIdx.x;
+  int next = 0, idle = 0;
+  uint32_t acc = 0u;
+  while (next < n_levels && idle < SP_IDLE) {
+    const int p = __hip_atomic_load(counters + CNT_STRAIN_PROG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p < 0) break;
+    if (p > next) next = min(p, n_levels);
+    const int target = min(n_levels, p + SP_AHEAD);
+    if (next >= target) {
+      ++idle;
+      __builtin_amdgcn_s_sleep(2);
+      continue;
+    }
+    idle = 0;
+    const int b = lvl_ptr[next], e = lvl_ptr[target];
+    // one word of every 128 B line of the records, and of the pre-check words
+    const uint32_t* rec = reinterpret_cast<const uint32_t*>(slots + b);
+    for (int i = tid; i < 2 * (e - b); i += blockDim.x) acc += rec[32 * (size_t)i];
+    for (int i = b + 32 * tid; i < e; i += 32 * (int)blockDim.x) acc += spre[i];
+    next = target;
+  }
+  // (keeps the loads: the sum is never this value in practice, and a match
+  // only writes an unused counter)
+  if (acc == 0x9e3779b9u && tid == 0) counters[CNT_STRAIN_PROG + 1] = (int)acc;
+}
+
 // Work entry of one iteration: bits 0-9 slot index within its level, bits
 // 10-13 sweep in flight, bits 14-17 marks.
 template <bool LP_LDS>
@@ -862,6 +905,10 @@ __global__ void __launch_bounds__(SP_THREADS)
   __shared__ int s_nwork[2], s_last_mark, s_state;
   __shared__ uint32_t s_any[SP_THREADS / 64];
   __shared__ int s_last_edges[SP_THREADS / 64], s_degen[SP_THREADS / 64];
+  if (blockIdx.x != 0) {  // the prefetcher (see strain_prefetch) and idle blocks
+    if (blockIdx.x == kNumXcd) strain_prefetch(lvl_ptr, n_levels, slots, spre, counters);
+    return;
+  }
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid == 0) {
     s_state = counters[CNT_STRAIN_FLAG];  // <0 finished, 0 no edge over limit, >0 sweeps needed
@@ -942,6 +989,7 @@ __global__ void __launch_bounds__(SP_THREADS)
   if (t < n_levels) fold_pre(t, pre_word(t));
   __syncthreads();
   while (t < tend) {
+    if (tid == 0) __hip_atomic_store(counters + CNT_STRAIN_PROG, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // --- scan: lane tid looks at slots 2 tid, 2 tid + 1 of every level in
     // flight.  Level offsets: lane 2k (2k + 1) reads the first (last) offset
     // of sweep k's level, broadcast with readlane; then one 16-bit LDS read
@@ -1110,6 +1158,7 @@ __global__ void __launch_bounds__(SP_THREADS)
     }
     t = tn;
   }
+  if (tid == 0) __hip_atomic_store(counters + CNT_STRAIN_PROG, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // totals
   {
     uint32_t any = e_any;
@@ -1160,11 +1209,11 @@ hipError_t strain_limit(hipStream_t st, DevState& S, double dt, int max_sweeps,
     if (marks && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
       (void)hipEventRecord(e0, st);
     if (S.n_levels <= SP_LP_MAX)
-      hipLaunchKernelGGL(k_strain_pipe<true>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
+      hipLaunchKernelGGL(k_strain_pipe<true>, dim3(kNumXcd + 1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
                          S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
                          S.counters);
     else
-      hipLaunchKernelGGL(k_strain_pipe<false>, dim3(1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
+      hipLaunchKernelGGL(k_strain_pipe<false>, dim3(kNumXcd + 1), dim3(SP_THREADS), 0, st, S.lvl_ptr,
                          S.n_levels, S.sslot, S.spre, S.lvl_active, dt, K, S.strain_delta,
                          S.counters);
     if (e1) {
