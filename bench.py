@@ -46,9 +46,9 @@ MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (2 * 128 * 2 + 2 * 128 * 128 + 128 * 24)  # 
 ROOFLINE_NOTES = {
     "strain": "one work-group runs about 2,900 dependent level iterations per step (the exact "
               "Gauss-Seidel order; a second work-group on its XCD reads the next levels' records into "
-              "L2); bound by that chain's latency, not by HBM: it moves about 1.9e7 B "
-              "per launch (profiles/traffic_strain.json) against the 1.6e8 B bound of reading every "
-              "vertex and element once",
+              "L2); bound by that chain's latency, not by HBM: it moves about 4.7e8 B "
+              "per launch (profiles/traffic_strain.json; mostly the prefetching work-group streaming the "
+              "records once) against the 1.6e8 B of reading every vertex and element once",
     "traverse": "per-lane stackless walks, bound by dependent L2 misses; walks longer than 512 steps "
                 "are continued over 64 sub-ranges by k_traverse_split (its time is included)",
     "narrow": "gather of each sub-test's four vertices; FP64 VALU co-bound",
