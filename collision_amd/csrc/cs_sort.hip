@@ -11,6 +11,8 @@
 //                      of lower matching lanes, and a per-wave LDS counter
 //                      carries counts across rounds.  Keys keep their input
 //                      order within a digit, so the sort is stable.
+#include <type_traits>
+
 #include "cs_internal.h"
 
 namespace cs {
@@ -324,7 +326,7 @@ constexpr int RANK_SORT_MAX = 1024;
 template <typename K>
 __global__ void __launch_bounds__(RANK_SORT_MAX) k_sort_rank(K* __restrict__ keys,
                                                              uint32_t* __restrict__ vals, int n) {
-  __shared__ K kb[RANK_SORT_MAX];
+  __shared__ __align__(16) K kb[RANK_SORT_MAX];
   const int i = threadIdx.x;
   const int w0 = i & ~63, w1 = min(n, w0 + 64);
   K ki = 0;
@@ -337,12 +339,31 @@ __global__ void __launch_bounds__(RANK_SORT_MAX) k_sort_rank(K* __restrict__ key
   __syncthreads();
   int rank = 0;
   if (i < n) {
-    for (int j = 0; j < w0; ++j) rank += kb[j] <= ki ? 1 : 0;  // j < i
+    // (w0 and w1 are multiples of 64 or n: the first and last loops read the
+    // keys 16 bytes at a time, several reads in flight)
+    constexpr int PER16 = 16 / (int)sizeof(K);
+    using V = typename std::conditional<sizeof(K) == 8, ulonglong2, uint4>::type;
+    const V* kv = reinterpret_cast<const V*>(kb);
+#pragma unroll 4
+    for (int j = 0; j < w0 / PER16; ++j) {  // keys before the wavefront: j < i
+      const V v = kv[j];
+      const K* e = reinterpret_cast<const K*>(&v);
+#pragma unroll
+      for (int u = 0; u < PER16; ++u) rank += e[u] <= ki ? 1 : 0;
+    }
     for (int j = w0; j < w1; ++j) {
       const K kj = kb[j];
       rank += (kj < ki || (kj == ki && j < i)) ? 1 : 0;
     }
-    for (int j = w1; j < n; ++j) rank += kb[j] < ki ? 1 : 0;  // j > i
+    int j = w1;
+#pragma unroll 4
+    for (; j + PER16 <= n; j += PER16) {  // keys after the wavefront: j > i
+      const V v = kv[j / PER16];
+      const K* e = reinterpret_cast<const K*>(&v);
+#pragma unroll
+      for (int u = 0; u < PER16; ++u) rank += e[u] < ki ? 1 : 0;
+    }
+    for (; j < n; ++j) rank += kb[j] < ki ? 1 : 0;
   }
   __syncthreads();  // every thread has read its key and value
   if (i < n) {
